@@ -1,0 +1,171 @@
+/*
+ * nbx.h -- C ABI of the MI355X (gfx950) force-evaluation and velocity-Verlet engine
+ * for NBodySimulator.jl systems.
+ *
+ * Reference interfaces replaced (paths relative to the NBodySimulator.jl tree):
+ *   - get_accelerating_function(p, sim) -> acceleration!(dv, u, v, t, i)
+ *       src/nbody_to_ode.jl:156-242; force laws src/basic_potentials.jl:240-433
+ *   - soode_system!(dv, v, u, p, t)        src/nbody_to_ode.jl:474-488, :502-532
+ *   - the VelocityVerlet step driven by run_simulation / calculate_simulation
+ *       src/nbody_simulation_result.jl:468-487 (+ OrdinaryDiffEqSymplecticRK)
+ *   - thermostats: berendsen_acceleration! src/thermostats.jl:76-91,
+ *       Andersen callback src/nbody_simulation_result.jl:504-540,
+ *       Langevin src/nbody_to_ode.jl:567-598
+ *   - energies / temperature / rdf          src/nbody_simulation_result.jl:182-446, :664-709
+ *
+ * Conventions
+ *   - Coordinate arrays cross the boundary as Julia column-major 3 x n Float64
+ *     matrices (x1 y1 z1 x2 ...), exactly the u/v/dv arrays the reference's closures
+ *     see.  Atom indices are 0-based (Julia's i is i+1).
+ *   - Every call is synchronous: host buffers may be reused on return.
+ *   - Return value: NBX_OK (0) or a negative NBX_ERR_* code; nbx_last_error(ctx)
+ *     gives the message (the reference throws BoundsError / KeyError / MethodError).
+ *   - One context = one HIP stream on the current device; a context is not to be
+ *     shared between threads without external locking (the reference is
+ *     single-threaded and non-reentrant as well).
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBX_OK 0
+#define NBX_ERR_INVALID (-1)     /* bad argument / unsupported combination        */
+#define NBX_ERR_BOUNDS (-2)      /* index outside 0..n-1 (reference: BoundsError)   */
+#define NBX_ERR_UNSUPPORTED (-3) /* feature not available for this system / mode    */
+#define NBX_ERR_DEVICE (-4)      /* HIP runtime error (no device, launch failure)   */
+#define NBX_ERR_CAPACITY (-5)    /* a cell or neighbour list overflowed its bound   */
+#define NBX_ERR_NONFINITE (-6)   /* a state value became NaN/Inf during a run       */
+
+/* Boundary conditions (src/boundary_conditions.jl:25-109) */
+#define NBX_BC_INFINITE 0  /* InfiniteBox()                                        */
+#define NBX_BC_CUBIC 1     /* CubicPeriodicBoundaryConditions(L): box[0] = L         */
+#define NBX_BC_PERIODIC 2  /* PeriodicBoundaryConditions: box = xmin xmax ymin ...   */
+
+/* System kinds (src/nbody_system.jl:72-75, :182-191) */
+#define NBX_SYS_POTENTIAL 0 /* PotentialNBodySystem (GravitationalSystem, ChargedParticles) */
+#define NBX_SYS_WATER 1     /* WaterSPCFw: atoms ordered O,H1,H2 per molecule (n = 3 nmol) */
+
+/* Precision modes */
+#define NBX_PREC_FP64_REF 0 /* FP64, all pairs in the reference's j order, no FMA
+                               contraction: bit-for-bit the reference's arithmetic   */
+#define NBX_PREC_MIXED 1    /* FP32 pair forces + Kahan accumulation, cell/Verlet
+                               lists (LJ, Coulomb, RF) or LDS tiles (gravity); FP64
+                               positions, velocities, integrator and reductions      */
+
+/* Thermostats (src/thermostats.jl:7-156) */
+#define NBX_THERM_NONE 0
+#define NBX_THERM_ANDERSEN 1  /* param = collision frequency nu */
+#define NBX_THERM_BERENDSEN 2 /* param = coupling time tau      */
+#define NBX_THERM_LANGEVIN 3  /* param = friction gamma         */
+
+typedef struct nbx_ctx nbx_ctx;
+
+/* Create a context for n coordinate columns.  masses: n values (get_masses order,
+ * src/nbody_simulation_result.jl:121-139); charges: n values or NULL.
+ * system_kind: NBX_SYS_*.  For NBX_SYS_WATER, n must be a multiple of 3.
+ * Replaces the host-side setup of src/nbody_to_ode.jl:263-375. */
+int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const double* masses,
+               const double* charges);
+int nbx_destroy(nbx_ctx* ctx);
+const char* nbx_last_error(const nbx_ctx* ctx);
+/* Library version string, and the number of HIP devices visible (0 without a GPU). */
+const char* nbx_version(void);
+int nbx_device_count(void);
+
+/* Boundary conditions: src/boundary_conditions.jl:25-109.  box: 6 doubles for
+ * PERIODIC, box[0] = L for CUBIC, ignored for INFINITE. */
+int nbx_set_boundary(nbx_ctx* ctx, int32_t kind, const double* box);
+/* Boltzmann constant of the unit system (NBodySimulation.kb, src/nbody_simulation.jl:42-54) */
+int nbx_set_kb(nbx_ctx* ctx, double kb);
+int nbx_set_precision(nbx_ctx* ctx, int32_t mode);
+
+/* Potential terms of a PotentialNBodySystem, applied per atom in the order added
+ * (the reference iterates its Dict, src/nbody_to_ode.jl:87-99).  For WaterSPCFw they
+ * set the system's lj_parameters / e_parameters (src/nbody_system.jl:182-191).  */
+/* LennardJonesParameters(eps, sigma, R): src/basic_potentials.jl:59-69, :240-272 */
+int nbx_add_lennard_jones(nbx_ctx* ctx, double eps, double sigma, double R);
+/* ElectrostaticParameters(k, R) (R = +Inf: no cutoff): src/basic_potentials.jl:147-163, :274-304 */
+int nbx_add_electrostatic(nbx_ctx* ctx, double k, double R);
+/* Reaction-field electrostatics (no reference counterpart; Tironi et al. 1995):
+ * eps_rf = dielectric constant beyond R (+Inf: conducting boundary). */
+int nbx_add_reaction_field(nbx_ctx* ctx, double k, double R, double eps_rf);
+/* GravitationalParameters(G): src/basic_potentials.jl:109-115, :306-331 */
+int nbx_add_gravitational(nbx_ctx* ctx, double G);
+/* SPCFwParameters(rOH, aHOH, kb, ka): src/basic_potentials.jl:233-238, :367-433 */
+int nbx_set_spcfw(nbx_ctx* ctx, double rOH, double aHOH, double kbond, double kangle);
+/* Thermostat (NBX_THERM_*), target T and its parameter (nu, tau or gamma). */
+int nbx_set_thermostat(nbx_ctx* ctx, int32_t kind, double T, double param);
+/* Neighbour-list skin for the mixed path, in length units (default: 0.12 x the
+ * largest cutoff, i.e. 0.3 sigma for R = 2.5 sigma).  The list is rebuilt on the
+ * device whenever an atom has moved more than skin/2 since the last build. */
+int nbx_set_skin(nbx_ctx* ctx, double skin);
+
+/* State: u, v are 3 x n column-major (gather_bodies_initial_coordinates,
+ * src/nbody_to_ode.jl:1-75).  get_state returns the current step's u and v. */
+int nbx_set_state(nbx_ctx* ctx, const double* u, const double* v);
+int nbx_get_state(const nbx_ctx* ctx, double* u, double* v);
+
+/* One right-hand-side evaluation (soode_system!, src/nbody_to_ode.jl:474-488 and
+ * :502-532): dv (3 x n) is OVERWRITTEN with the sum, per atom and in the reference's
+ * order, of every potential closure, the SPC/Fw bond/angle group terms and the
+ * Berendsen thermostat term.  u, v: host arrays (v is read only for Berendsen). */
+int nbx_rhs(nbx_ctx* ctx, const double* u, const double* v, double* dv);
+
+/* The per-atom closure protocol (acceleration!(dv, u, v, t, i) of one potential,
+ * src/nbody_to_ode.jl:156-242): ADDS into dv (3 x n) the contribution of potential
+ * term `term` (0-based, in the order added) for every atom i in [i_begin, i_end).
+ * For water, term 0 = electrostatic, 1 = Lennard-Jones (O atoms only),
+ * 2 = SPC/Fw bonds, 3 = SPC/Fw angles (group closure, src/nbody_to_ode.jl:244-261). */
+int nbx_accelerations(nbx_ctx* ctx, int32_t term, const double* u, int64_t i_begin,
+                      int64_t i_end, double* dv);
+
+/* Velocity-Verlet run from the context state (OrdinaryDiffEqSymplecticRK
+ * VelocityVerlet, called from src/nbody_simulation_result.jl:480-487):
+ *   u' = u + dt v + dt^2 (a/2);  vh = v + dt (a/2);  a' = f(u', vh);  v' = vh + dt (a'/2)
+ * with thermostats applied per step (Andersen after the step; Langevin as half
+ * Ornstein-Uhlenbeck steps around it; Berendsen inside f).  Random numbers come from
+ * Philox4x32-10 keyed by `seed` with counter (atom, absolute step), so a run split
+ * into pieces reproduces an unsplit one bit for bit.
+ * Frames: if save_every > 0, frame 0 is the initial state and frame k the state
+ * after k*save_every steps; u_frames/v_frames hold 3n doubles per frame
+ * (1 + nsteps/save_every frames).  Pass NULL frames to keep everything in HBM.
+ * On return the context state is the final state. */
+int nbx_vv_run(nbx_ctx* ctx, double dt, int64_t nsteps, uint64_t seed, int64_t save_every,
+               double* u_frames, double* v_frames);
+/* Steps taken by all nbx_vv_run calls since the last set_state (solution.stats.naccept). */
+int64_t nbx_steps_taken(const nbx_ctx* ctx);
+
+/* Energies of the current state, in FP64 on the device:
+ *   ke = kinetic_energy (src/nbody_simulation_result.jl:209-218)
+ *   pe = potential_energy with the reference's conventions (:239-403): LJ tail for
+ *        far pairs, Coulomb q_j/R beyond R, SPC/Fw bonds and angles; gravity adds
+ *        nothing there, so it is returned separately in pe_grav (may be NULL). */
+int nbx_energies(nbx_ctx* ctx, double* ke, double* pe, double* pe_grav);
+/* md_temperature of the current state (src/thermostats.jl:87-91 with the system's
+ * degrees of freedom, src/nbody_simulation_result.jl:152-164). */
+int nbx_temperature(nbx_ctx* ctx, double* T);
+
+/* rdf (src/nbody_simulation_result.jl:664-709) over nframes position frames
+ * (each 3 x n, host memory), cubic box only; rs and gr receive maxbin values. */
+int nbx_rdf(nbx_ctx* ctx, const double* frames, int64_t nframes, int64_t maxbin,
+            double* rs, double* gr);
+
+/* Measurement support: average duration (ms) of the step's dominant kernel over
+ * `iters` back-to-back launches on the context's stream, timed with HIP events on
+ * that stream, on the current state (which is left unchanged).  *kernel_name
+ * receives a static string naming the kernel. */
+int nbx_time_force_kernel(nbx_ctx* ctx, int32_t iters, double* avg_ms,
+                          const char** kernel_name);
+/* Ordered in-cutoff pairs of the current state (pair-interaction count of one force
+ * evaluation; n(n-1) for gravity). */
+int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBX_H */

@@ -1,0 +1,131 @@
+// nbx_gravity.hip -- all-pairs gravity (src/basic_potentials.jl:306-331) in mixed
+// precision on gfx950.
+//
+// Layout: pos = float4 (x, y, z, m) per body, written by the drift kernel from the FP64
+// state.  Grid = (target blocks) x (source chunks): a block owns TPT*256 targets and
+// one contiguous chunk of sources; it stages 256 sources at a time in LDS and every
+// lane reads the same LDS entry (a broadcast read) per inner iteration.  Within a tile
+// the per-target sums are plain FP32; tile sums are Kahan-added into the per-target
+// accumulator, and chunk partials are stored in FP64 and added by the kick kernel in
+// chunk order (deterministic).
+//   a_i = G * sum_j m_j (r_j - r_i) / |r_j - r_i|^3      (= -G m_j r_ij / |r_ij|^3)
+// The self term is removed by clamping r^2 to 1e-20 before the reciprocal square root:
+// for j == i the displacement is exactly zero, so its contribution is exactly zero.
+#include "nbx_mixed.h"
+
+#define GR_THREADS 256
+#define GR_TILE 256
+
+template <int TPT>
+__global__ void __launch_bounds__(GR_THREADS)
+k_grav_tiled(const float4* __restrict__ pos, int n, int jchunk, double* __restrict__ part) {
+    __shared__ float4 tile[GR_TILE];
+    const int tid = threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int ibase = blockIdx.x * (GR_THREADS * TPT) + tid;
+    float xi[TPT], yi[TPT], zi[TPT];
+    nbx_kahan ax[TPT], ay[TPT], az[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = ibase + t * GR_THREADS;
+        const float4 p = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        xi[t] = p.x; yi[t] = p.y; zi[t] = p.z;
+        ax[t].init(); ay[t].init(); az[t].init();
+    }
+    const int j0 = chunk * jchunk;
+    const int j1 = (j0 + jchunk) < n ? (j0 + jchunk) : n;
+    for (int base = j0; base < j1; base += GR_TILE) {
+        const int j = base + tid;
+        tile[tid] = j < j1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        float sx[TPT], sy[TPT], sz[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) { sx[t] = 0.f; sy[t] = 0.f; sz[t] = 0.f; }
+#pragma unroll 8
+        for (int q = 0; q < GR_TILE; ++q) {
+            const float4 pj = tile[q];
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                const float dx = pj.x - xi[t], dy = pj.y - yi[t], dz = pj.z - zi[t];
+                float r2 = dx * dx + dy * dy + dz * dz;
+                r2 = fmaxf(r2, 1e-20f);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float s = pj.w * (rinv * rinv * rinv);
+                sx[t] += dx * s;
+                sy[t] += dy * s;
+                sz[t] += dz * s;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) { ax[t].add(sx[t]); ay[t].add(sy[t]); az[t].add(sz[t]); }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = ibase + t * GR_THREADS;
+        if (i < n) {
+            double* pc = part + (size_t)chunk * 3 * n;
+            pc[i] = ax[t].value();
+            pc[n + i] = ay[t].value();
+            pc[2 * n + i] = az[t].value();
+        }
+    }
+}
+
+// a_i = G * sum over chunks (chunk order), then the end of the step
+__global__ void k_grav_finish(const double* __restrict__ part, int chunks, int n, double G,
+                              double* __restrict__ a, double* __restrict__ v,
+                              const double* __restrict__ vh, const double* __restrict__ ms,
+                              mx_kick_params kp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < chunks; ++c) {
+        const double* pc = part + (size_t)c * 3 * n;
+        s[0] += pc[i];
+        s[1] += pc[n + i];
+        s[2] += pc[2 * n + i];
+    }
+    double ax = G * s[0], ay = G * s[1], az = G * s[2];
+    if (kp.therm == NBX_THERM_BERENDSEN) {
+        const double cb = mx_berendsen_coeff(kp);
+        ax += cb * vh[i]; ay += cb * vh[n + i]; az += cb * vh[2 * n + i];
+    }
+    a[i] = ax; a[n + i] = ay; a[2 * n + i] = az;
+    if (!kp.do_kick) return;
+    mx_finish_atom(i, n, ax, ay, az, vh, v, ms, kp);
+    if (kp.therm == NBX_THERM_ANDERSEN &&
+        nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
+        mx_andersen_atom(i, n, kp.m0, v, kp);
+}
+
+static int grav_tpt(int64_t n) { return n >= 16384 ? 2 : 1; }
+
+void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, double G, const mx_kick_params& kp,
+                       int force_only_kernel) {
+    const int n = (int)c->n;
+    if (n <= 0) return;
+    const int tpt = grav_tpt(n);
+    const int chunks = c->grav_chunks;
+    const int jchunk = (int)((n + chunks - 1) / chunks);
+    const dim3 grid(nbx_blocks(n, GR_THREADS * tpt), chunks);
+    if (tpt == 2)
+        hipLaunchKernelGGL(k_grav_tiled<2>, grid, dim3(GR_THREADS), 0, s,
+                           (const float4*)c->d_pos, n, jchunk, c->d_part);
+    else
+        hipLaunchKernelGGL(k_grav_tiled<1>, grid, dim3(GR_THREADS), 0, s,
+                           (const float4*)c->d_pos, n, jchunk, c->d_part);
+    if (force_only_kernel) return;
+    hipLaunchKernelGGL(k_grav_finish, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, c->d_part,
+                       chunks, n, G, c->d_a, c->d_v, c->d_vh, c->d_ms, kp);
+}
+
+// number of source chunks: enough blocks to fill 256 CUs several times over, with
+// chunks of at least 2048 sources
+int mx_gravity_chunks(int64_t n) {
+    const int tpt = grav_tpt(n);
+    const int64_t tblocks = (n + GR_THREADS * tpt - 1) / (GR_THREADS * tpt);
+    int chunks = 1;
+    while (tblocks * chunks < 2048 && (n / (chunks * 2)) >= 2048) chunks *= 2;
+    return chunks;
+}

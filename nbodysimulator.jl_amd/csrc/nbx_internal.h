@@ -1,0 +1,148 @@
+// nbx_internal.h -- context layout and kernel-launcher interfaces shared by the
+// translation units of libnbx.so.  Device state is SoA FP64: component k of atom i
+// lives at x[k * n + i].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "nbx.h"
+#include "nbx_device.h"
+
+enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4 };
+
+struct nbx_term {
+    int kind;
+    double p[4];  // LJ: eps sigma R; ELEC: k R; GRAV: G; RF: k R eps_rf
+};
+
+// Mixed-path neighbour machinery (cell list + Verlet list)
+struct nbx_nlist {
+    bool ready = false;
+    int lanes = 8;            // lanes per atom in the force kernel (G)
+    int cap = 0;              // neighbour slots per atom (multiple of lanes)
+    int ncell = 0;            // cells per side (0: all-pairs neighbour search)
+    int ccap = 0;             // atoms per cell bound
+    double rlist = 0.0;       // list radius = cutoff + skin
+    double cutoff = 0.0;      // largest interaction cutoff
+    double skin = 0.0;        // skin in use (rebuild when an atom moved skin/2)
+    int* d_nbr = nullptr;     // neighbour slots, interleaved by lanes (see nbx_nonbonded.hip)
+    int* d_nnb = nullptr;     // neighbour counts
+    int* d_cell_count = nullptr;
+    int* d_cell_atoms = nullptr;
+    double* d_xref = nullptr; // unwrapped positions at the last build (SoA)
+    int* d_flags = nullptr;   // [0] rebuild request, [1] overflow, [2] builds done
+    size_t cap_bytes = 0;
+};
+
+struct nbx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int sys = NBX_SYS_POTENTIAL;
+    int prec = NBX_PREC_FP64_REF;
+    std::vector<double> h_ms, h_qs;
+    bool has_q = false;
+    nbx_bcv bc{};
+    double kb = 1.38e-23;  // kb_SI, src/nbody_simulation.jl:7
+    std::vector<nbx_term> terms;                   // potential system, in order
+    nbx_term water_lj{}, water_elec{};             // water system
+    bool water_lj_set = false, water_elec_set = false;
+    double rOH = 0, aHOH = 0, kbond = 0, kangle = 0;
+    bool spcfw_set = false;
+    int therm = NBX_THERM_NONE;
+    double therm_T = 0, therm_p = 0;
+    double skin = 0.0;        // user skin (when skin_set), else 0.12 x the largest cutoff
+    bool skin_set = false;
+    // device state
+    double* d_x = nullptr;    // positions (unwrapped), 3n
+    double* d_v = nullptr;    // velocities, 3n
+    double* d_a = nullptr;    // accelerations at d_x, 3n
+    double* d_vh = nullptr;   // half-kicked velocities, 3n
+    double* d_tmp = nullptr;  // scratch 3n (host transfers, rhs)
+    double* d_ms = nullptr;
+    double* d_qs = nullptr;
+    double* d_red = nullptr;  // reduction scratch (n + 64 doubles)
+    double* d_scal = nullptr; // device scalars: [0] temperature for Berendsen
+    void* d_pos = nullptr;    // mixed path: double4 wrapped positions (+charge) or float4 (+mass)
+    double* d_part = nullptr; // gravity: per-chunk partial accelerations
+    int grav_chunks = 1;
+    nbx_nlist nl;
+    bool state_set = false;
+    bool accel_valid = false; // d_a holds f(d_x, d_v) for the current state
+    int64_t steps_taken = 0;
+    std::string err;
+};
+
+// error helpers ----------------------------------------------------------------
+int nbx_fail(nbx_ctx* c, int code, const std::string& msg);
+#define NBX_HIP(c, call)                                                           \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return nbx_fail((c), NBX_ERR_DEVICE,                                   \
+                            std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+static inline unsigned nbx_blocks(int64_t work, int threads) {
+    return (unsigned)((work + threads - 1) / threads);
+}
+
+// ---- FP64 reference-order kernels (nbx_ref64.hip, built with -ffp-contract=off) ----
+// Each ADDS its contribution into dv (SoA) for atoms i in [ib, ie) (stride as noted).
+void ref64_fill(hipStream_t s, double* p, int64_t count, double value);
+void ref64_aos_to_soa(hipStream_t s, const double* aos, double* soa, int64_t n);
+void ref64_soa_to_aos(hipStream_t s, const double* soa, double* aos, int64_t n);
+// LJ: i = ib, ib+istride, ... < ie; j = 0, jstride, ..., (nj-1)*jstride
+void ref64_lj(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
+              int64_t ib, int64_t ie, int istride, int jstride, int64_t nj, double eps,
+              double sigma, double R, const nbx_bcv& bc);
+void ref64_elec(hipStream_t s, const double* x, double* dv, const double* ms,
+                const double* qs, int64_t n, int64_t ib, int64_t ie, int water, double k,
+                double R, const nbx_bcv& bc);
+void ref64_rf(hipStream_t s, const double* x, double* dv, const double* ms, const double* qs,
+              int64_t n, int64_t ib, int64_t ie, int water, double k, double R,
+              double eps_rf, const nbx_bcv& bc);
+void ref64_grav(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
+                int64_t ib, int64_t ie, double G);
+void ref64_bond(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
+                int64_t ib, int64_t ie, double rOH, double kbond);
+// molecules m in [mb, me)
+void ref64_angle(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
+                 int64_t mb, int64_t me, double aHOH, double ka);
+// T = sum m v^2 / (kb (3N - Nc)) into *d_T (sequential order), then dv += ... (Berendsen)
+void ref64_temperature(hipStream_t s, const double* v, const double* ms, int64_t n,
+                       double kb, int64_t Nc, double* d_tmp, double* d_T);
+void ref64_berendsen(hipStream_t s, double* dv, const double* v, int64_t n, const double* d_T,
+                     double T0, double gamma);
+// VV pieces
+void ref64_drift(hipStream_t s, double* x, const double* v, const double* a, double* vh,
+                 int64_t n, double dt);
+void ref64_kick(hipStream_t s, double* v, const double* vh, const double* a, int64_t n,
+                double dt);
+void ref64_langevin_half(hipStream_t s, double* v, const double* ms, int64_t n, double kb,
+                         double T, double gamma, double dt, uint64_t seed, int64_t step,
+                         uint32_t stream);
+void ref64_andersen(hipStream_t s, double* v, const double* ms, int64_t n, int water,
+                    double kb, double T, double prob, uint64_t seed, int64_t step);
+// energies: per-term sums into *d_out (FP64, sequential final sum)
+void ref64_kinetic(hipStream_t s, const double* v, const double* ms, int64_t n, double* d_tmp,
+                   double* d_out);
+void ref64_lj_pe(hipStream_t s, const double* x, int64_t n, int jstride, int64_t nj, double eps,
+                 double sigma, double R, const nbx_bcv& bc, double* d_tmp, double* d_out);
+void ref64_elec_pe(hipStream_t s, const double* x, const double* qs, int64_t n, int water,
+                   double k, double R, int rf, double eps_rf, const nbx_bcv& bc,
+                   double* d_tmp, double* d_out);
+void ref64_grav_pe(hipStream_t s, const double* x, const double* ms, int64_t n, double G,
+                   double* d_tmp, double* d_out);
+void ref64_bonded_pe(hipStream_t s, const double* x, int64_t n, double rOH, double kbond,
+                     double aHOH, double ka, double* d_tmp, double* d_out2);
+void ref64_rdf_hist(hipStream_t s, const double* frame_aos, int64_t n, int jstride,
+                    int64_t nj, double L, int64_t maxbin, unsigned long long* d_hist);
+int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d_flag);
+
+// ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----
+int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name);
+int mx_count_pairs(nbx_ctx* c, int64_t* pairs);
+void mx_release(nbx_ctx* c);

@@ -1,0 +1,24 @@
+"""NBodySimulator.jl's API over the gfx950 engine libnbx (include/nbx.h).
+
+Same names, fields and argument meaning as the Julia package (module export list,
+src/NBodySimulator.jl:26-36); the force evaluation, the velocity-Verlet step, the
+thermostats, energies, temperature and rdf run as HIP kernels on the GPU.
+"""
+from ._lib import LIB_PATH, Context, NBXError, device_count, load
+from .simulation import (MI355XVerlet, SimulationResult, Solution, VelocityVerlet,
+                         build_context, get_accelerating_function,
+                         get_group_accelerating_function, get_position, get_velocity,
+                         gravitational_potential_energy, initial_energy, kinetic_energy,
+                         potential_energy, rdf, register_accelerating_function, run_simulation,
+                         temperature, total_energy)
+from .types import (AndersenThermostat, BerendsenThermostat, ChargedParticle, ChargedParticles,
+                    CubicPeriodicBoundaryConditions, ElectrostaticParameters,
+                    GravitationalParameters, GravitationalSystem, InfiniteBox,
+                    LangevinThermostat, LennardJonesParameters, MagneticParticle,
+                    MagnetostaticParameters, MassBody, NBodySimulation, NoseHooverThermostat,
+                    NullThermostat, PeriodicBoundaryConditions, PotentialNBodySystem,
+                    PotentialParameters, ReactionFieldParameters, SPCFwParameters,
+                    WaterMolecule, WaterSPCFw, get_masses, kb_SI)
+from .generators import generate_bodies_in_cell_nodes
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,265 @@
+"""ctypes binding of libnbx.so (include/nbx.h).
+
+The library is loaded from ``nbodysimulator.jl_amd/lib/libnbx.so`` (built in-tree by
+``nbodysimulator.jl_amd/build.py``).  There is no fallback: if the library is
+missing, every entry point raises ``ImportError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libnbx.so")
+
+NBX_OK = 0
+NBX_ERR_INVALID = -1
+NBX_ERR_BOUNDS = -2
+NBX_ERR_UNSUPPORTED = -3
+NBX_ERR_DEVICE = -4
+NBX_ERR_CAPACITY = -5
+NBX_ERR_NONFINITE = -6
+
+BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
+SYS_POTENTIAL, SYS_WATER = 0, 1
+PREC_FP64_REF, PREC_MIXED = 0, 1
+THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN = 0, 1, 2, 3
+
+_P = C.c_void_p
+_D = C.c_double
+_I32 = C.c_int32
+_I64 = C.c_int64
+_U64 = C.c_uint64
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol here is declared in include/nbx.h
+SIGNATURES = {
+    "nbx_create": (C.c_int, [C.POINTER(_P), _I64, _I32, _DP, _DP]),
+    "nbx_destroy": (C.c_int, [_P]),
+    "nbx_last_error": (C.c_char_p, [_P]),
+    "nbx_version": (C.c_char_p, []),
+    "nbx_device_count": (C.c_int, []),
+    "nbx_set_boundary": (C.c_int, [_P, _I32, _DP]),
+    "nbx_set_kb": (C.c_int, [_P, _D]),
+    "nbx_set_precision": (C.c_int, [_P, _I32]),
+    "nbx_add_lennard_jones": (C.c_int, [_P, _D, _D, _D]),
+    "nbx_add_electrostatic": (C.c_int, [_P, _D, _D]),
+    "nbx_add_reaction_field": (C.c_int, [_P, _D, _D, _D]),
+    "nbx_add_gravitational": (C.c_int, [_P, _D]),
+    "nbx_set_spcfw": (C.c_int, [_P, _D, _D, _D, _D]),
+    "nbx_set_thermostat": (C.c_int, [_P, _I32, _D, _D]),
+    "nbx_set_skin": (C.c_int, [_P, _D]),
+    "nbx_set_state": (C.c_int, [_P, _DP, _DP]),
+    "nbx_get_state": (C.c_int, [_P, _DP, _DP]),
+    "nbx_rhs": (C.c_int, [_P, _DP, _DP, _DP]),
+    "nbx_accelerations": (C.c_int, [_P, _I32, _DP, _I64, _I64, _DP]),
+    "nbx_vv_run": (C.c_int, [_P, _D, _I64, _U64, _I64, _DP, _DP]),
+    "nbx_steps_taken": (_I64, [_P]),
+    "nbx_energies": (C.c_int, [_P, _DP, _DP, _DP]),
+    "nbx_temperature": (C.c_int, [_P, _DP]),
+    "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
+    "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
+    "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libnbx.so once; raise ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libnbx.so not found at {LIB_PATH}; build it with "
+            "`python nbodysimulator.jl_amd/build.py` (the HIP engine has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class NBXError(RuntimeError):
+    """A libnbx call failed; ``code`` is the NBX_ERR_* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def raise_for(code: int, msg: str):
+    if code == NBX_ERR_BOUNDS:
+        raise IndexError(msg)          # Julia: BoundsError
+    if code == NBX_ERR_INVALID:
+        raise ValueError(msg)          # Julia: ArgumentError / MethodError
+    if code == NBX_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if code == NBX_ERR_NONFINITE:
+        raise FloatingPointError(msg)
+    raise NBXError(code, msg)
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_DP)
+
+
+def as_c3n(a, n: int) -> np.ndarray:
+    """Coordinates as a C-contiguous float64 (n, 3) array: the memory layout of a Julia
+    column-major 3 x n matrix."""
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if arr.shape == (3, n) and n != 3:
+        arr = np.ascontiguousarray(arr.T)
+    if arr.shape != (n, 3):
+        raise ValueError(f"expected coordinates of shape ({n}, 3), got {arr.shape}")
+    return arr
+
+
+class Context:
+    """Owns one nbx_ctx (one HIP stream on the current device)."""
+
+    def __init__(self, n: int, system_kind: int, masses, charges=None):
+        lib = load()
+        self._lib = lib
+        self.n = int(n)
+        self.masses = np.ascontiguousarray(np.asarray(masses, dtype=np.float64))
+        self.charges = (None if charges is None else
+                        np.ascontiguousarray(np.asarray(charges, dtype=np.float64)))
+        h = _P()
+        rc = lib.nbx_create(C.byref(h), self.n, system_kind, dptr(self.masses),
+                            dptr(self.charges))
+        if rc != NBX_OK:
+            raise_for(rc, lib.nbx_last_error(None).decode())
+        self._h = h
+
+    # -- plumbing -------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != NBX_OK:
+            raise_for(rc, self._lib.nbx_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration ---------------------------------------------------------
+    def set_boundary(self, kind: int, box=None):
+        b = np.zeros(6) if box is None else np.ascontiguousarray(np.asarray(box, np.float64))
+        self._check(self._lib.nbx_set_boundary(self._h, kind, dptr(b)))
+
+    def set_kb(self, kb: float):
+        self._check(self._lib.nbx_set_kb(self._h, float(kb)))
+
+    def set_precision(self, mode: int):
+        self._check(self._lib.nbx_set_precision(self._h, mode))
+
+    def add_lennard_jones(self, eps, sigma, R):
+        self._check(self._lib.nbx_add_lennard_jones(self._h, float(eps), float(sigma), float(R)))
+
+    def add_electrostatic(self, k, R):
+        self._check(self._lib.nbx_add_electrostatic(self._h, float(k), float(R)))
+
+    def add_reaction_field(self, k, R, eps_rf):
+        self._check(self._lib.nbx_add_reaction_field(self._h, float(k), float(R), float(eps_rf)))
+
+    def add_gravitational(self, G):
+        self._check(self._lib.nbx_add_gravitational(self._h, float(G)))
+
+    def set_spcfw(self, rOH, aHOH, kbond, kangle):
+        self._check(self._lib.nbx_set_spcfw(self._h, float(rOH), float(aHOH), float(kbond),
+                                            float(kangle)))
+
+    def set_thermostat(self, kind: int, T: float = 0.0, param: float = 1.0):
+        self._check(self._lib.nbx_set_thermostat(self._h, kind, float(T), float(param)))
+
+    def set_skin(self, skin: float):
+        self._check(self._lib.nbx_set_skin(self._h, float(skin)))
+
+    # -- state and evaluation ------------------------------------------------------
+    def set_state(self, u, v=None):
+        u = as_c3n(u, self.n)
+        v = None if v is None else as_c3n(v, self.n)
+        self._check(self._lib.nbx_set_state(self._h, dptr(u), dptr(v)))
+
+    def get_state(self):
+        u = np.empty((self.n, 3))
+        v = np.empty((self.n, 3))
+        self._check(self._lib.nbx_get_state(self._h, dptr(u), dptr(v)))
+        return u, v
+
+    def rhs(self, u, v=None) -> np.ndarray:
+        u = as_c3n(u, self.n)
+        v = None if v is None else as_c3n(v, self.n)
+        dv = np.empty((self.n, 3))
+        self._check(self._lib.nbx_rhs(self._h, dptr(u), dptr(v), dptr(dv)))
+        return dv
+
+    def accelerations(self, term: int, u, i_begin: int, i_end: int, dv: np.ndarray):
+        """Add term `term`'s contribution for atoms [i_begin, i_end) into dv (n, 3)."""
+        u = as_c3n(u, self.n)
+        if dv.dtype != np.float64 or not dv.flags.c_contiguous or dv.shape != (self.n, 3):
+            raise ValueError("dv must be a C-contiguous float64 (n, 3) array")
+        self._check(self._lib.nbx_accelerations(self._h, int(term), dptr(u), int(i_begin),
+                                                int(i_end), dptr(dv)))
+        return dv
+
+    def vv_run(self, dt: float, nsteps: int, seed: int = 0, save_every: int = 0):
+        nframes = 1 + nsteps // save_every if save_every > 0 else 0
+        uf = vf = None
+        if nframes:
+            uf = np.empty((nframes, self.n, 3))
+            vf = np.empty((nframes, self.n, 3))
+        self._check(self._lib.nbx_vv_run(self._h, float(dt), int(nsteps), int(seed),
+                                          int(save_every), dptr(uf), dptr(vf)))
+        return uf, vf
+
+    def steps_taken(self) -> int:
+        return int(self._lib.nbx_steps_taken(self._h))
+
+    def energies(self):
+        ke, pe, pg = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._lib.nbx_energies(self._h, C.byref(ke), C.byref(pe), C.byref(pg)))
+        return ke.value, pe.value, pg.value
+
+    def temperature(self) -> float:
+        t = C.c_double()
+        self._check(self._lib.nbx_temperature(self._h, C.byref(t)))
+        return t.value
+
+    def rdf(self, frames: np.ndarray, maxbin: int = 1000):
+        f = np.ascontiguousarray(np.asarray(frames, dtype=np.float64))
+        if f.ndim != 3 or f.shape[1:] != (self.n, 3):
+            raise ValueError("frames must have shape (nframes, n, 3)")
+        rs = np.empty(maxbin)
+        gr = np.empty(maxbin)
+        self._check(self._lib.nbx_rdf(self._h, dptr(f), f.shape[0], maxbin, dptr(rs), dptr(gr)))
+        return rs, gr
+
+    def time_force_kernel(self, iters: int):
+        ms = C.c_double()
+        name = C.c_char_p()
+        self._check(self._lib.nbx_time_force_kernel(self._h, int(iters), C.byref(ms),
+                                                    C.byref(name)))
+        return ms.value, name.value.decode()
+
+    def count_pairs(self) -> int:
+        p = C.c_int64()
+        self._check(self._lib.nbx_count_pairs(self._h, C.byref(p)))
+        return p.value
+
+
+def device_count() -> int:
+    return int(load().nbx_device_count())

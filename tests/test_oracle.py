@@ -1,0 +1,390 @@
+"""Pins the FP64 CPU oracle (oracle/nbx_oracle.c) without running Julia.
+
+Each check is chosen so that a plausible restatement mistake (dropped term, wrong sign,
+index or operand order) fails it: textbook closed forms, F = -grad U against the
+oracle's own energy functions (a separate code path), Newton's third law, extended-
+precision brute force, the reference's own closed-form tests (test/*.jl) and
+published known-answer vectors for Philox4x32-10.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def lj_system(n, bc=None, eps=1.0, sigma=1.0, R=2.5, m=1.0):
+    s = O.System(np.full(n, m), bc=bc)
+    return s.add(O.TERM_LJ, eps, sigma, R)
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions (src/boundary_conditions.jl:111-172)
+# ---------------------------------------------------------------------------
+
+def test_periodic_bc_wraps_into_lo_hi_not_minimum_image():
+    bc = O.make_bc(O.BC_PERIODIC, [0, 1, 0, 1, 0, 1])
+    rij, r, r2 = O.interparticle_distance([0.1, 0.5, 0.95], [0.4, 0.3, 0.05], bc)
+    np.testing.assert_allclose(rij, [0.7, 0.2, 0.9], rtol=0, atol=1e-15)
+    assert r2 == pytest.approx(0.49 + 0.04 + 0.81, rel=1e-15)
+
+
+def test_cubic_bc_is_half_open_minimum_image():
+    bc = O.make_bc(O.BC_CUBIC, 1.0)
+    rij, _, _ = O.interparticle_distance([0.6, -0.5, 0.5], [0, 0, 0], bc)
+    # x >= L/2 wraps down, x = -L/2 stays, x = +L/2 wraps to -L/2
+    np.testing.assert_array_equal(rij, [0.6 - 1.0, -0.5, -0.5])
+    rij, _, _ = O.interparticle_distance([100.25, -3.25, 0.0], [0, 0, 0], bc)
+    np.testing.assert_allclose(rij, [0.25, -0.25, 0.0], atol=1e-12)
+
+
+def test_infinite_box_plain_difference():
+    rij, r, r2 = O.interparticle_distance([3.0, 4.0, 0.0], [0, 0, 0], O.make_bc())
+    assert r == 5.0 and r2 == 25.0
+    np.testing.assert_array_equal(rij, [3.0, 4.0, 0.0])
+
+
+# ---------------------------------------------------------------------------
+# Force laws against closed forms
+# ---------------------------------------------------------------------------
+
+def test_lj_two_body_matches_minus_dU_dr():
+    r = 1.3
+    s = lj_system(2, R=math.inf)
+    u = np.array([[-r / 2, 0, 0], [r / 2, 0, 0]])
+    a = s.rhs(u)
+    # U = 4 eps ((s/r)^12 - (s/r)^6);  F_0 = dU/dr * x_hat (pulls atom 0 towards +x if attractive)
+    dUdr = 4.0 * (-12.0 / r ** 13 + 6.0 / r ** 7)
+    np.testing.assert_allclose(a[0], [dUdr, 0, 0], rtol=1e-14)
+    np.testing.assert_allclose(a[1], [-dUdr, 0, 0], rtol=1e-14)
+
+
+def test_lj_cutoff_and_mass_scaling():
+    s = O.System(np.array([2.0, 1.0])).add(O.TERM_LJ, 1.5, 1.1, 1.2)
+    a = s.rhs(np.array([[0, 0, 0], [1.25, 0, 0]]))
+    np.testing.assert_array_equal(a, 0.0)  # beyond R
+    a = s.rhs(np.array([[0, 0, 0], [1.15, 0, 0]]))
+    r = 1.15
+    F = 4 * 1.5 * (12 * 1.1 ** 12 / r ** 13 - 6 * 1.1 ** 6 / r ** 7)
+    np.testing.assert_allclose(a[:, 0], [-F / 2.0, F / 1.0], rtol=1e-13)
+
+
+def test_coulomb_two_body():
+    k, q1, q2, m1, m2 = 9.0e9, 1e-3, -2e-3, 1.0, 3.0
+    s = O.System(np.array([m1, m2]), charges=np.array([q1, q2])).add(O.TERM_ELEC, k, math.inf)
+    a = s.rhs(np.array([[0, 0, 0], [0, 2.0, 0]]))
+    F = k * q1 * q2 / 4.0  # along -y for atom 0 when attractive
+    np.testing.assert_allclose(a[0], [0, -F / m1, 0], rtol=1e-14)
+    np.testing.assert_allclose(a[1], [0, F / m2, 0], rtol=1e-14)
+
+
+def test_gravity_two_body():
+    s = O.System(np.array([2.0, 5.0])).add(O.TERM_GRAV, 0.7)
+    a = s.rhs(np.array([[0, 0, 0], [0, 0, 3.0]]))
+    np.testing.assert_allclose(a[0], [0, 0, 0.7 * 5.0 / 9.0], rtol=1e-14)
+    np.testing.assert_allclose(a[1], [0, 0, -0.7 * 2.0 / 9.0], rtol=1e-14)
+
+
+def _fd_forces(energy, u, h=1e-6):
+    g = np.zeros_like(u)
+    for i in range(u.shape[0]):
+        for k in range(3):
+            up = u.copy(); up[i, k] += h
+            dn = u.copy(); dn[i, k] -= h
+            g[i, k] = -(energy(up) - energy(dn)) / (2 * h)
+    return g
+
+
+def _rand_config(n, L, seed, dmin=0.9):
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(0, L, 3)
+        ok = True
+        for q in pts:
+            d = p - q
+            d -= L * np.round(d / L)
+            if np.linalg.norm(d) < dmin:
+                ok = False
+        if ok:
+            pts.append(p)
+    return np.array(pts)
+
+
+def _away_from_cutoff(u, L, R, margin=1e-3):
+    n = len(u)
+    for i in range(n):
+        for j in range(i + 1, n):
+            d = u[i] - u[j]
+            d -= L * np.round(d / L)
+            if abs(np.linalg.norm(d) - R) < margin:
+                return False
+    return True
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_lj_force_is_minus_gradient_of_energy(seed):
+    L, R = 5.0, 2.2
+    u = _rand_config(12, L, seed)
+    assert _away_from_cutoff(u, L, R)
+    s = lj_system(12, bc=O.make_bc(O.BC_CUBIC, L), R=R, m=1.7)
+    a = s.rhs(u)
+    F = _fd_forces(s.potential_energy, u)
+    np.testing.assert_allclose(a * 1.7, F, rtol=1e-5, atol=1e-6 * np.abs(F).max())
+
+
+def test_coulomb_and_rf_forces_are_minus_gradient():
+    L = 6.0
+    u = _rand_config(10, L, 5)
+    q = np.array([1, -1] * 5, dtype=float)
+    m = np.linspace(1, 2, 10)
+    bc = O.make_bc(O.BC_CUBIC, L)
+    for kind, params in ((O.TERM_ELEC, (2.0, 2.5)), (O.TERM_RF, (2.0, 2.5, 78.5)),
+                         (O.TERM_RF, (2.0, 2.5, math.inf))):
+        assert _away_from_cutoff(u, L, 2.5)
+        s = O.System(m, charges=q, bc=bc).add(kind, *params)
+        a = s.rhs(u)
+        F = _fd_forces(s.potential_energy, u)
+        np.testing.assert_allclose(a * m[:, None], F, rtol=1e-5, atol=1e-7 * np.abs(F).max())
+
+
+def test_gravity_force_is_minus_gradient():
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(9, 3))
+    m = rng.uniform(0.5, 2, 9)
+    s = O.System(m).add(O.TERM_GRAV, 1.3)
+    a = s.rhs(u)
+    F = _fd_forces(lambda x: O.grav_potential(x, m, 1.3), u)
+    np.testing.assert_allclose(a * m[:, None], F, rtol=1e-6, atol=1e-8)
+
+
+def water_system(nmol, L, seed, elec_kind=O.TERM_ELEC, jitter=0.02):
+    from nbodysimulator.generators import SPCFW as P
+    rng = np.random.default_rng(seed)
+    side = int(round(nmol ** (1 / 3)))
+    a = L / side
+    g = (np.arange(side) + 0.5) * a
+    cx, cy, cz = np.meshgrid(g, g, g, indexing="ij")
+    Opos = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1)[:nmol]
+    from nbodysimulator.generators import random_rotations
+    R = random_rotations(nmol, rng)
+    h1 = np.array([P["rOH"] * 1.03, 0, 0])
+    th = P["aHOH"] * 0.95
+    h2 = np.array([P["rOH"] * 0.97 * math.cos(th), P["rOH"] * 0.97 * math.sin(th), 0])
+    u = np.empty((3 * nmol, 3))
+    u[0::3] = Opos
+    u[1::3] = Opos + R @ h1
+    u[2::3] = Opos + R @ h2
+    u += jitter * 0.1 * rng.standard_normal(u.shape)
+    ms = np.tile([P["mO"], P["mH"], P["mH"]], nmol)
+    qs = np.tile([P["qO"], P["qH"], P["qH"]], nmol)
+    elec = (P["k_el"], 0.45 * L) if elec_kind == O.TERM_ELEC else (P["k_el"], 0.45 * L, math.inf)
+    s = O.System(ms, charges=qs, water=True, bc=O.make_bc(O.BC_CUBIC, L), kb=P["kb"])
+    s.water_params((P["eps"], P["sigma"], 0.45 * L), elec_kind, elec, P["rOH"], P["aHOH"],
+                   P["kbond"], P["kangle"])
+    return s, u, ms
+
+
+@pytest.mark.parametrize("elec_kind", [O.TERM_ELEC, O.TERM_RF])
+def test_water_rhs_is_minus_gradient_of_water_energy(elec_kind):
+    s, u, ms = water_system(8, 1.0, 11, elec_kind)
+    a = s.rhs(u)
+    F = _fd_forces(s.potential_energy, u, h=1e-7)
+    np.testing.assert_allclose(a * ms[:, None], F, rtol=2e-5, atol=2e-5 * np.abs(F).max())
+
+
+def test_newton_third_law_cubic_and_infinite():
+    for bc in (O.make_bc(O.BC_CUBIC, 7.0), O.make_bc()):
+        u = _rand_config(40, 7.0, 9, dmin=0.95)
+        m = np.random.default_rng(1).uniform(1, 3, 40)
+        s = O.System(m, bc=bc).add(O.TERM_LJ, 1.0, 1.0, 2.5)
+        tot = (s.rhs(u) * m[:, None]).sum(axis=0)
+        scale = np.abs(s.rhs(u) * m[:, None]).sum()
+        assert np.abs(tot).max() <= 1e-13 * scale
+
+
+def test_lj_matches_extended_precision_brute_force():
+    L, R = 6.0, 2.5
+    u = _rand_config(24, L, 21, dmin=0.85)
+    s = lj_system(24, bc=O.make_bc(O.BC_CUBIC, L), R=R)
+    a = s.rhs(u)
+    ul = u.astype(np.longdouble)
+    ref = np.zeros_like(ul)
+    for i in range(24):
+        d = ul[i] - ul
+        d -= np.longdouble(L) * np.round(d / np.longdouble(L))
+        r2 = (d * d).sum(axis=1)
+        r2[i] = np.inf
+        inr = r2 < R * R
+        s6 = (1 / r2[inr]) ** 3
+        ref[i] = (24 * ((2 * s6 * s6 - s6) / r2[inr])[:, None] * d[inr]).sum(axis=0)
+    err = np.abs(a - ref.astype(np.float64)).max() / np.abs(ref).max()
+    assert err < 1e-13
+
+
+def test_angle_force_alone_is_minus_gradient():
+    # a single molecule: the only forces are bond + angle; remove the bond term by
+    # putting both O-H lengths at rOH, so only the angle acts
+    from nbodysimulator.generators import SPCFW as P
+    ms = np.array([P["mO"], P["mH"], P["mH"]])
+    qs = np.zeros(3)
+    s = O.System(ms, charges=qs, water=True, kb=P["kb"])
+    s.water_params((P["eps"], P["sigma"], 1.0), O.TERM_ELEC, (0.0, 1.0), P["rOH"], P["aHOH"],
+                   P["kbond"], P["kangle"])
+    th = P["aHOH"] + 0.2
+    r0 = P["rOH"]
+    u = np.array([[0, 0, 0], [r0, 0, 0], [r0 * math.cos(th), r0 * math.sin(th), 0.0]])
+    a = s.rhs(u)
+    # analytic: U = ka/2 (th - th0)^2; torque-free in-plane forces of magnitude dU/dth / r0
+    dU = P["kangle"] * 0.2
+    f1 = a[1] * ms[1]
+    assert np.linalg.norm(f1) == pytest.approx(dU / r0, rel=1e-12)
+    assert f1[1] > 0  # H1 on +x is pushed towards H2 (angle shrinks)
+    np.testing.assert_allclose((a * ms[:, None]).sum(axis=0), 0, atol=1e-9 * dU)
+    F = _fd_forces(s.potential_energy, u, h=1e-8)
+    np.testing.assert_allclose(a * ms[:, None], F, rtol=1e-5, atol=1e-6 * np.abs(F).max())
+
+
+# ---------------------------------------------------------------------------
+# Velocity Verlet: the reference's closed-form tests
+# ---------------------------------------------------------------------------
+
+def test_lj_two_particles_speed_from_energy():  # test/lennard_jones_test.jl:4-34
+    r1 = 1.3
+    s = lj_system(2, R=math.inf)
+    u0 = np.array([[-r1 / 2, 0, 0], [r1 / 2, 0, 0]])
+    u, v, _ = s.vv_run(u0, np.zeros((2, 3)), 1e-3, 1000)
+    r2 = np.linalg.norm(u[1] - u[0])
+    v_exp = math.sqrt(4 * ((1 / r1) ** 12 - (1 / r2) ** 12 - ((1 / r1) ** 6 - (1 / r2) ** 6)))
+    assert np.linalg.norm(v[1]) == pytest.approx(v_exp, rel=1e-3)
+
+
+FIG8_U = np.array([[-0.995492, 0, 0], [0.995492, 0, 0], [0, 0, 0.0]])
+FIG8_V = np.array([[-0.347902, -0.53393, 0], [-0.347902, -0.53393, 0], [0.695804, 1.06786, 0.0]])
+
+
+def test_figure_eight_returns_after_period():  # test/gravitational_test.jl:233-240
+    s = O.System(np.ones(3)).add(O.TERM_GRAV, 1.0)
+    u, v, _ = s.vv_run(FIG8_U, FIG8_V, math.pi / 130, 260)
+    np.testing.assert_allclose(u, FIG8_U, atol=1e-3)
+    assert s.kinetic_energy(FIG8_V) == pytest.approx(1.218, abs=1e-3)
+
+
+def test_two_charges_repel_speed_from_energy():  # test/electrostatics_test.jl:42-68
+    k, q = 9.0e9, 1.0e-3
+    s = O.System(np.ones(2), charges=np.array([q, q])).add(O.TERM_ELEC, k, math.inf)
+    u0 = np.array([[-0.5, 0, 0], [0.5, 0, 0]])
+    u, v, _ = s.vv_run(u0, np.zeros((2, 3)), 1e-3, 1000)
+    r2 = np.linalg.norm(u[1] - u[0])
+    v_exp = math.sqrt(k * q * q * (1 / 1.0 - 1 / r2))
+    assert np.linalg.norm(v[1]) == pytest.approx(v_exp, rel=1e-3)
+
+
+def test_eight_charges_energy_conservation():  # test/electrostatics_test.jl:84-119
+    n, L, m, q, k = 8, 1.0, 1.0, 1.0, 9.0e9
+    dL = L / (math.ceil(n ** (1 / 3)) + 1)
+    pts = []
+    xs = np.arange(dL / 2, L + 1e-12, dL)
+    for x in xs:
+        for y in xs:
+            for z in xs:
+                if len(pts) < n:
+                    pts.append([x, y, z])
+    u0 = np.array(pts)
+    tau = 0.01 * dL / math.sqrt(2 * k * q * q / (dL * m))
+    s = O.System(np.full(n, m), charges=np.full(n, q), bc=O.make_bc(O.BC_CUBIC, L))
+    s.add(O.TERM_ELEC, k, 0.45 * L)
+    u, v, _ = s.vv_run(u0, np.zeros((n, 3)), tau, 1000)
+    e1 = s.potential_energy(u0)
+    e2 = s.potential_energy(u) + s.kinetic_energy(v)
+    assert abs((e2 - e1) / e1) < 1e-3
+
+
+def _kepler_error(dt):
+    s = O.System(np.array([1.0, 1e-9])).add(O.TERM_GRAV, 1.0)
+    u0 = np.array([[0, 0, 0], [1.0, 0, 0]])
+    v0 = np.array([[0, 0, 0], [0, 1.0, 0]])
+    T = 2 * math.pi
+    n = int(round(T / dt))
+    u, _, _ = s.vv_run(u0, v0, T / n, n)
+    return np.linalg.norm(u[1] - u0[1])
+
+
+def test_vv_is_second_order():
+    e1, e2 = _kepler_error(2 * math.pi / 200), _kepler_error(2 * math.pi / 400)
+    assert 3.6 < e1 / e2 < 4.4
+
+
+def test_berendsen_relaxation_of_free_particles():
+    n, T0, Ti, tau = 200, 2.0, 1.0, 0.05
+    rng = np.random.default_rng(0)
+    v0 = rng.standard_normal((n, 3))
+    m = np.ones(n)
+    v0 *= math.sqrt(Ti / ((v0 * v0).sum() / (3 * n)))
+    s = O.System(m).thermostat(O.THERM_BERENDSEN, T0, tau)
+    dt, steps = 1e-4, 1000
+    u, v, _ = s.vv_run(np.zeros((n, 3)), v0, dt, steps)
+    T = (v * v).sum() / (3 * n)
+    expect = T0 + (Ti - T0) * math.exp(-steps * dt / tau)
+    assert T == pytest.approx(expect, rel=2e-3)
+
+
+@pytest.mark.parametrize("kind,param", [(O.THERM_ANDERSEN, 50.0), (O.THERM_LANGEVIN, 5.0)])
+def test_stochastic_thermostats_reach_target_temperature(kind, param):
+    n, T0 = 400, 1.7
+    m = np.full(n, 2.0)
+    s = O.System(m).thermostat(kind, T0, param)
+    u, v, _ = s.vv_run(np.zeros((n, 3)), np.zeros((n, 3)), 0.01, 400, seed=5)
+    T = (m[:, None] * v * v).sum() / (3 * n)
+    assert T == pytest.approx(T0, rel=0.08)
+    # Maxwellian: <v^4> / <v^2>^2 = 3 per component
+    k4 = (v ** 4).mean() / (v ** 2).mean() ** 2
+    assert k4 == pytest.approx(3.0, rel=0.15)
+
+
+def test_kinetic_energy_and_temperature_of_argon_triplet():  # test/lennard_jones_test.jl:47-82
+    T, kb, m = 120.0, 8.3144598e-3, 39.95
+    vd = math.sqrt(3 * kb * T / m)
+    v = np.array([[0, 0, -vd], [0, -vd, 0], [-vd, 0, 0]])
+    ms = np.full(3, m)
+    lib = O.load()
+    assert lib.orc_md_temperature(O.dp(v), O.dp(ms), kb, 3, 0) == pytest.approx(120.0, abs=1e-6)
+    ke = O.System(ms).kinetic_energy(v)
+    assert ke == pytest.approx(m * 3 * vd * vd / 2, rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# RDF (src/nbody_simulation_result.jl:664-709)
+# ---------------------------------------------------------------------------
+
+def test_rdf_bin_convention():
+    L = 1.0
+    f = np.array([[[0.1, 0.1, 0.1], [0.35, 0.1, 0.1]]])   # r = 0.25 -> bin ceil(2.5) = 3
+    rs, gr = O.rdf(f, L, maxbin=10)
+    nideal = 4 / 3 * math.pi * 2 / L ** 3 * (0.3 ** 3 - 0.2 ** 3)
+    assert gr[2] == pytest.approx((2 / (1 * 2)) / nideal, rel=1e-12)
+    assert np.count_nonzero(gr) == 1
+    np.testing.assert_allclose(rs, (np.arange(10) + 0.5) * 0.1, rtol=1e-12)
+
+
+def test_rdf_of_ideal_gas_is_one():
+    rng = np.random.default_rng(0)
+    frames = rng.uniform(0, 4.0, (20, 300, 3))
+    rs, gr = O.rdf(frames, 4.0, maxbin=20)
+    sel = (rs > 0.8) & (rs < 1.9)
+    assert np.abs(gr[sel] - 1).max() < 0.08
+
+
+# ---------------------------------------------------------------------------
+# Philox4x32-10 known-answer vectors (Random123 kat_vectors)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_known_answers(ctr, key, expect):
+    assert tuple(O.philox(ctr, key)) == expect
