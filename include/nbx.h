@@ -164,6 +164,10 @@ int nbx_time_force_kernel(nbx_ctx* ctx, int32_t iters, double* avg_ms,
 /* Ordered in-cutoff pairs of the current state (pair-interaction count of one force
  * evaluation; n(n-1) for gravity). */
 int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
+/* Diagnostics of the mixed path (no reference counterpart): out[0] Verlet-list builds
+ * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),
+ * [3] lanes per atom in the force kernel, [4] steps per captured graph.  nout <= 5. */
+int nbx_get_stats(nbx_ctx* ctx, int64_t* out, int32_t nout);
 
 #ifdef __cplusplus
 }

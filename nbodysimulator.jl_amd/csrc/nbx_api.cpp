@@ -5,7 +5,9 @@
 // the velocity-Verlet step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -77,6 +79,11 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     }
     c->bc.kind = NBX_BC_INFINITE;
     for (int k = 0; k < 6; ++k) c->bc.b[k] = (k % 2 == 0) ? -INFINITY : INFINITY;
+    if (const char* g = std::getenv("NBX_GRAPH_STEPS")) c->graph_steps = std::max(0, std::atoi(g));
+    if (const char* g = std::getenv("NBX_LANES")) {
+        const int l = std::atoi(g);
+        if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16) c->nl.lanes = l;
+    }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     const size_t b3 = sizeof(double) * (size_t)(3 * (n > 0 ? n : 1));
@@ -116,6 +123,7 @@ extern "C" int nbx_destroy(nbx_ctx* c) {
 static void invalidate(nbx_ctx* c) {
     c->accel_valid = false;
     c->nl.ready = false;
+    ++c->version;
 }
 
 extern "C" int nbx_set_boundary(nbx_ctx* c, int32_t kind, const double* box) {
@@ -661,6 +669,19 @@ extern "C" int nbx_time_force_kernel(nbx_ctx* c, int32_t iters, double* avg_ms,
     if (c->prec != NBX_PREC_MIXED)
         return nbx_fail(c, NBX_ERR_UNSUPPORTED, "kernel timing is provided for the mixed path");
     return mx_time_force(c, iters, avg_ms, name);
+}
+
+extern "C" int nbx_get_stats(nbx_ctx* c, int64_t* out, int32_t nout) {
+    if (!c || !out || nout < 0 || nout > 5) return NBX_ERR_INVALID;
+    int64_t v[5] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps};
+    if (c->nl.d_flags) {
+        int h[8] = {0};
+        NBX_HIP(c, hipMemcpyAsync(h, c->nl.d_flags, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        NBX_HIP(c, hipStreamSynchronize(c->stream));
+        v[0] = h[2];
+    }
+    for (int k = 0; k < nout; ++k) out[k] = v[k];
+    return NBX_OK;
 }
 
 extern "C" int nbx_count_pairs(nbx_ctx* c, int64_t* pairs) {

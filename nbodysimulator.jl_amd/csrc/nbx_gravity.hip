@@ -72,37 +72,44 @@ k_grav_tiled(const float4* __restrict__ pos, int n, int jchunk, double* __restri
     }
 }
 
-// a_i = G * sum over chunks (chunk order), then the end of the step
-__global__ void k_grav_finish(const double* __restrict__ part, int chunks, int n, double G,
-                              double* __restrict__ a, double* __restrict__ v,
-                              const double* __restrict__ vh, const double* __restrict__ ms,
-                              mx_kick_params kp) {
+// a_i = G * sum over chunks (chunk order), then the end of the step and the drift
+__global__ void __launch_bounds__(256)
+k_grav_finish(const double* __restrict__ part, int chunks, int n, double G, mx_kick_params kp,
+              mx_io io) {
+    mx_resolve_step(kp);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < chunks; ++c) {
-        const double* pc = part + (size_t)c * 3 * n;
-        s[0] += pc[i];
-        s[1] += pc[n + i];
-        s[2] += pc[2 * n + i];
+    double tval = 0.0;
+    if (i < n) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int c = 0; c < chunks; ++c) {
+            const double* pc = part + (size_t)c * 3 * n;
+            s[0] += pc[i];
+            s[1] += pc[n + i];
+            s[2] += pc[2 * n + i];
+        }
+        double ax = G * s[0], ay = G * s[1], az = G * s[2];
+        if (kp.therm == NBX_THERM_BERENDSEN) {
+            const double cb = mx_berendsen_coeff(kp);
+            ax += cb * io.vh[i]; ay += cb * io.vh[n + i]; az += cb * io.vh[2 * n + i];
+        }
+        io.a[i] = ax; io.a[n + i] = ay; io.a[2 * n + i] = az;
+        if (kp.do_kick) {
+            double v3[3];
+            mx_kick_atom(i, n, ax, ay, az, io.vh, io.ms, kp, v3);
+            if (kp.therm == NBX_THERM_ANDERSEN &&
+                nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
+                mx_andersen_atom(i, kp.m0, kp, v3);
+            io.v[i] = v3[0]; io.v[n + i] = v3[1]; io.v[2 * n + i] = v3[2];
+            if (io.do_drift) tval = mx_drift_atom(i, n, v3, ax, ay, az, kp, io);
+        }
     }
-    double ax = G * s[0], ay = G * s[1], az = G * s[2];
-    if (kp.therm == NBX_THERM_BERENDSEN) {
-        const double cb = mx_berendsen_coeff(kp);
-        ax += cb * vh[i]; ay += cb * vh[n + i]; az += cb * vh[2 * n + i];
-    }
-    a[i] = ax; a[n + i] = ay; a[2 * n + i] = az;
-    if (!kp.do_kick) return;
-    mx_finish_atom(i, n, ax, ay, az, vh, v, ms, kp);
-    if (kp.therm == NBX_THERM_ANDERSEN &&
-        nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
-        mx_andersen_atom(i, n, kp.m0, v, kp);
+    mx_epilogue(tval, io, n);
 }
 
 static int grav_tpt(int64_t n) { return n >= 16384 ? 2 : 1; }
 
-void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, double G, const mx_kick_params& kp,
-                       int force_only_kernel) {
+void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double G,
+                       const mx_kick_params& kp, const mx_io& io, int force_only_kernel) {
     const int n = (int)c->n;
     if (n <= 0) return;
     const int tpt = grav_tpt(n);
@@ -110,14 +117,14 @@ void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, double G, const mx_kick_
     const int jchunk = (int)((n + chunks - 1) / chunks);
     const dim3 grid(nbx_blocks(n, GR_THREADS * tpt), chunks);
     if (tpt == 2)
-        hipLaunchKernelGGL(k_grav_tiled<2>, grid, dim3(GR_THREADS), 0, s,
-                           (const float4*)c->d_pos, n, jchunk, c->d_part);
+        hipLaunchKernelGGL(k_grav_tiled<2>, grid, dim3(GR_THREADS), 0, s, (const float4*)pos, n,
+                           jchunk, c->d_part);
     else
-        hipLaunchKernelGGL(k_grav_tiled<1>, grid, dim3(GR_THREADS), 0, s,
-                           (const float4*)c->d_pos, n, jchunk, c->d_part);
+        hipLaunchKernelGGL(k_grav_tiled<1>, grid, dim3(GR_THREADS), 0, s, (const float4*)pos, n,
+                           jchunk, c->d_part);
     if (force_only_kernel) return;
     hipLaunchKernelGGL(k_grav_finish, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, c->d_part,
-                       chunks, n, G, c->d_a, c->d_v, c->d_vh, c->d_ms, kp);
+                       chunks, n, G, kp, io);
 }
 
 // number of source chunks: enough blocks to fill 256 CUs several times over, with

@@ -29,11 +29,29 @@ struct nbx_nlist {
     double skin = 0.0;        // skin in use (rebuild when an atom moved skin/2)
     int* d_nbr = nullptr;     // neighbour slots, interleaved by lanes (see nbx_nonbonded.hip)
     int* d_nnb = nullptr;     // neighbour counts
-    int* d_cell_count = nullptr;
-    int* d_cell_atoms = nullptr;
+    int* d_cell_count = nullptr;  // 2 x ncells (bins of even / odd steps)
+    int* d_cell_atoms = nullptr;  // 2 x ncells x ccap
     double* d_xref = nullptr; // unwrapped positions at the last build (SoA)
-    int* d_flags = nullptr;   // [0] rebuild request, [1] overflow, [2] builds done
+    int* d_flags = nullptr;   // [0] / [3] rebuild request for even / odd steps,
+                              // [1] overflow, [2] builds done, [4] reduction counter
     size_t cap_bytes = 0;
+};
+
+// What a captured step graph depends on; a mismatch forces a new capture.
+struct nbx_graph_key {
+    int valid = 0;
+    int64_t version = 0;      // configuration version (bumped by every setter)
+    double dt = 0.0;
+    uint64_t seed = 0;
+    const void* nbr = nullptr;
+    const void* pos = nullptr;
+    const void* part = nullptr;
+    int cap = 0, ccap = 0, ncell = 0, lanes = 0, steps = 0;
+    bool operator==(const nbx_graph_key& o) const {
+        return valid && o.valid && version == o.version && dt == o.dt && seed == o.seed &&
+               nbr == o.nbr && pos == o.pos && part == o.part && cap == o.cap &&
+               ccap == o.ccap && ncell == o.ncell && lanes == o.lanes && steps == o.steps;
+    }
 };
 
 struct nbx_ctx {
@@ -66,9 +84,16 @@ struct nbx_ctx {
     double* d_red = nullptr;  // reduction scratch (n + 64 doubles)
     double* d_scal = nullptr; // device scalars: [0] temperature for Berendsen
     void* d_pos = nullptr;    // mixed path: double4 wrapped positions (+charge) or float4 (+mass)
+    void* d_pos2 = nullptr;   // second position buffer (odd steps)
+    double* d_x2 = nullptr;   // second unwrapped-position buffer (odd steps)
     double* d_part = nullptr; // gravity: per-chunk partial accelerations
     int grav_chunks = 1;
     nbx_nlist nl;
+    int64_t* d_step = nullptr;       // absolute step base read by replayed graphs
+    hipGraphExec_t gexec = nullptr;  // captured block of mixed-path steps
+    nbx_graph_key gkey;
+    int64_t version = 0;             // bumped by every configuration change
+    int graph_steps = 50;            // steps per captured graph (0: no graphs)
     bool state_set = false;
     bool accel_valid = false; // d_a holds f(d_x, d_v) for the current state
     int64_t steps_taken = 0;

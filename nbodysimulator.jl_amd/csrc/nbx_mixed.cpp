@@ -1,8 +1,14 @@
-// nbx_mixed.cpp -- host orchestration of the mixed-precision path: plan selection
-// (Verlet list / all pairs / gravity tiles / water), list sizing, the step sequence
-//   drift -> [temperature] -> list rebuild (device-gated) -> force + end of step
-// and overflow recovery (a chunk of steps is replayed from a device-side backup with
-// larger lists; the counter-based RNG makes the replay identical).
+// nbx_mixed.cpp -- host orchestration of the mixed-precision path.
+//
+// Plans: Verlet list (cubic box, finite cutoffs), all pairs (any other boundary or an
+// infinite cutoff), LDS-tiled gravity, SPC/Fw water (non-bonded + per-molecule kernel).
+// A run is a sequence of chunks (<= 500 steps, or up to the next saved frame); each
+// chunk starts from x in X[0] with a drift kernel, then enqueues per step
+//     build(k) [device-gated] ; step(k)      (water: step = non-bonded + molecule)
+// either through a captured hipGraph of graph_steps steps (an even count, so buffer
+// parities repeat) replayed with a device-side step base, or directly for the rest.
+// A chunk that overflows a cell or neighbour list is replayed from a device backup with
+// larger lists; the counter-based RNG makes the replay identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -107,7 +113,13 @@ void free_nl(nbx_nlist& nl) {
 
 int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
-// size and allocate the lists for the current configuration (cap/ccap may be grown)
+int alloc_flags(nbx_ctx* c) {
+    NBX_HIP(c, hipMalloc((void**)&c->nl.d_flags, sizeof(int) * 8));
+    NBX_HIP(c, hipMemsetAsync(c->nl.d_flags, 0, sizeof(int) * 8, c->stream));
+    return NBX_OK;
+}
+
+// size and allocate the lists for the current configuration (cap/ccap grown by `grow`)
 int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     nbx_nlist& nl = c->nl;
     const int lanes = nl.lanes;
@@ -121,24 +133,28 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     const double rho = (double)n / (L * L * L);
     const double est = 4.0 / 3.0 * M_PI * nl.rlist * nl.rlist * nl.rlist * rho;
     int cap = (int)std::ceil(grow * (1.6 * est + 24.0));
-    cap = std::min<int64_t>(cap, std::max<int64_t>(n, 1));
+    cap = (int)std::min<int64_t>(cap, std::max<int64_t>(n, 1));
     nl.cap = round_up(std::max(cap, 1), lanes);
     int nc = (int)std::floor(L / nl.rlist);
     if (nc < 3) nc = 0;
-    if (nc > 256) nc = 256;
+    if (nc > 128) nc = 128;
     nl.ncell = nc;
     const double per_cell = nc ? (double)n / ((double)nc * nc * nc) : 0.0;
     nl.ccap = nc ? (int)std::ceil(grow * (2.0 * per_cell + 16.0)) : 0;
+    if (nl.ccap > 1024)
+        return nbx_fail(c, NBX_ERR_CAPACITY, "cell occupancy above 1024 atoms (box too dense)");
     const size_t ncells = (size_t)nc * nc * nc;
     NBX_HIP(c, hipMalloc((void**)&nl.d_nbr, sizeof(int) * (size_t)nl.cap * (size_t)n));
     NBX_HIP(c, hipMalloc((void**)&nl.d_nnb, sizeof(int) * (size_t)n));
-    NBX_HIP(c, hipMalloc((void**)&nl.d_cell_count, sizeof(int) * std::max<size_t>(ncells, 1)));
+    NBX_HIP(c, hipMalloc((void**)&nl.d_cell_count, sizeof(int) * 2 * std::max<size_t>(ncells, 1)));
     NBX_HIP(c, hipMalloc((void**)&nl.d_cell_atoms,
-                         sizeof(int) * std::max<size_t>(ncells * (size_t)nl.ccap, 1)));
+                         sizeof(int) * 2 * std::max<size_t>(ncells * (size_t)nl.ccap, 1)));
     NBX_HIP(c, hipMalloc((void**)&nl.d_xref, sizeof(double) * 3 * (size_t)n));
-    NBX_HIP(c, hipMalloc((void**)&nl.d_flags, sizeof(int) * 8));
-    NBX_HIP(c, hipMemsetAsync(nl.d_flags, 0, sizeof(int) * 8, c->stream));
+    int rc = alloc_flags(c);
+    if (rc) return rc;
     NBX_HIP(c, hipMemsetAsync(nl.d_nnb, 0, sizeof(int) * (size_t)n, c->stream));
+    NBX_HIP(c, hipMemsetAsync(nl.d_cell_count, 0, sizeof(int) * 2 * std::max<size_t>(ncells, 1),
+                              c->stream));
     nl.cap_bytes = sizeof(int) * (size_t)nl.cap * (size_t)n;
     nl.ready = true;
     return NBX_OK;
@@ -149,8 +165,7 @@ int pos_kind(const mx_plan& p) {
     return p.nb.mode_list ? 1 : 0;
 }
 
-mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t seed,
-                           int64_t step) {
+mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t seed) {
     mx_kick_params kp{};
     kp.do_kick = do_kick;
     kp.therm = c->therm;
@@ -158,7 +173,6 @@ mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t se
     kp.kb = c->kb;
     kp.T = c->therm_T;
     kp.seed = seed;
-    kp.step = step;
     kp.m0 = c->n > 0 ? c->h_ms[0] : 1.0;
     kp.d_T = c->d_scal;  // [0]
     if (c->therm == NBX_THERM_LANGEVIN) {
@@ -174,23 +188,92 @@ mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t se
 
 int64_t constraints(const nbx_ctx* c) { return c->sys == NBX_SYS_WATER ? 2 * (c->n / 3) : 0; }
 
-// force evaluation at the current positions, optionally finishing the step
-void launch_force(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp) {
-    hipStream_t s = c->stream;
-    if (p.kind == PLAN_GRAVITY) {
-        mx_launch_gravity(s, c, p.G, kp, 0);
-        return;
+double* xbuf(nbx_ctx* c, int par) { return par ? c->d_x2 : c->d_x; }
+void* pbuf(nbx_ctx* c, int par) { return par ? c->d_pos2 : c->d_pos; }
+int* req_flag(nbx_ctx* c, int par) { return c->nl.d_flags + (par ? 3 : 0); }
+int* bin_count(nbx_ctx* c, int par) {
+    const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
+    return c->nl.d_cell_count ? c->nl.d_cell_count + par * ncells : nullptr;
+}
+int* bin_atoms(nbx_ctx* c, int par) {
+    const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
+    return c->nl.d_cell_atoms ? c->nl.d_cell_atoms + par * ncells * (size_t)c->nl.ccap : nullptr;
+}
+
+// buffers of a launch that reads x in X[par] and (when drifting) writes X[par ^ 1]
+mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift) {
+    mx_io io{};
+    io.x_in = xbuf(c, par);
+    io.x_out = xbuf(c, par ^ 1);
+    io.pos_kind = pos_kind(p);
+    io.pos_out = pbuf(c, par ^ 1);
+    io.a = c->d_a;
+    io.v = c->d_v;
+    io.vh = c->d_vh;
+    io.ms = c->d_ms;
+    io.qs = c->has_q ? c->d_qs : nullptr;
+    io.do_drift = do_drift;
+    io.flags = c->nl.d_flags;
+    const bool list = p.kind != PLAN_GRAVITY && p.nb.mode_list;
+    if (list) {
+        io.xref = c->nl.d_xref;
+        io.half_skin2 = 0.25 * c->nl.skin * c->nl.skin;
+        io.req_out = req_flag(c, par ^ 1);
+        io.L = c->bc.b[0];
     }
-    if (p.nb.mode_list) mx_launch_rebuild(s, c, p.nb.water);
-    mx_nb_launch nb = p.nb;
-    nb.lanes = c->nl.lanes;
-    mx_launch_nonbonded(s, nb, c, p.pp, kp, p.nb.mode_list ? c->nl.d_flags : nullptr);
-    if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp);
+    io.need_T = (c->therm == NBX_THERM_BERENDSEN && do_drift) ? 1 : 0;
+    io.red_part = c->d_red;
+    io.red_counter = reinterpret_cast<unsigned*>(c->nl.d_flags + 4);
+    io.T_out = c->d_scal;
+    io.T_denom = c->kb * (double)(3 * c->n - constraints(c));
+    return io;
+}
+
+// after a kernel that wrote x into X[par_out]: bins for a requested list build, and the
+// Berendsen temperature from the kernel's `nparts` per-block partials
+void after_drift(nbx_ctx* c, const mx_plan& p, int par_out, int nparts) {
+    hipStream_t s = c->stream;
+    if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3)
+        mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
+                      bin_atoms(c, par_out));
+    if (c->therm == NBX_THERM_BERENDSEN)
+        mx_launch_T_final(s, c->d_red, nparts, c->kb * (double)(3 * c->n - constraints(c)),
+                          c->d_scal);
+}
+
+// blocks of the kernel that ends a step (its per-block temperature partials)
+int step_blocks(const nbx_ctx* c, const mx_plan& p) {
+    if (p.kind == PLAN_WATER) return (int)nbx_blocks(c->n / 3, 128);
+    if (p.kind == PLAN_NONBONDED && p.nb.mode_list) return (int)nbx_blocks(c->n * c->nl.lanes, 256);
+    return (int)nbx_blocks(c->n, 256);
+}
+
+// force evaluation at x in X[par] (list: P[par]); with kp.do_kick the step's end, with
+// do_drift the start of the next step
+void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par, int do_drift) {
+    hipStream_t s = c->stream;
+    const mx_io io = make_io(c, p, par, do_drift);
+    if (p.kind == PLAN_GRAVITY) {
+        mx_launch_gravity(s, c, pbuf(c, par), p.G, kp, io, 0);
+    } else {
+        mx_nb_launch nb = p.nb;
+        nb.lanes = c->nl.lanes;
+        int* clear = p.nb.mode_list ? req_flag(c, par) : nullptr;
+        mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io, clear);
+        if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
+    }
+    if (do_drift) after_drift(c, p, par ^ 1, step_blocks(c, p));
+}
+
+void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
+    if (p.kind == PLAN_GRAVITY || !p.nb.mode_list) return;
+    mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
+                    bin_count(c, par), bin_atoms(c, par),
+                    c->nl.ncell >= 3 ? bin_count(c, par ^ 1) : nullptr);
 }
 
 struct backup4 {
     double* p = nullptr;
-    size_t n3 = 0;
     ~backup4() { if (p) (void)hipFree(p); }
 };
 
@@ -205,9 +288,12 @@ static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
     std::string why;
     if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
     const int64_t n = c->n;
-    if (!c->d_pos) {
-        NBX_HIP(c, hipMalloc(&c->d_pos, sizeof(double) * 4 * (size_t)n));
-    }
+    const size_t pb = sizeof(double) * 4 * (size_t)std::max<int64_t>(n, 1);
+    if (!c->d_pos) NBX_HIP(c, hipMalloc(&c->d_pos, pb));
+    if (!c->d_pos2) NBX_HIP(c, hipMalloc(&c->d_pos2, pb));
+    if (!c->d_x2)
+        NBX_HIP(c, hipMalloc((void**)&c->d_x2, sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1)));
+    if (!c->d_step) NBX_HIP(c, hipMalloc((void**)&c->d_step, sizeof(int64_t)));
     if (p.kind == PLAN_GRAVITY) {
         const int chunks = mx_gravity_chunks(n);
         if (!c->d_part || c->grav_chunks != chunks) {
@@ -216,16 +302,11 @@ static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
             NBX_HIP(c, hipMalloc((void**)&c->d_part, sizeof(double) * 3 * (size_t)n * chunks));
             c->grav_chunks = chunks;
         }
-        free_nl(c->nl);
-        NBX_HIP(c, hipMalloc((void**)&c->nl.d_flags, sizeof(int) * 8));
-        NBX_HIP(c, hipMemsetAsync(c->nl.d_flags, 0, sizeof(int) * 8, c->stream));
-        c->nl.ready = true;
-        return NBX_OK;
     }
-    if (!p.nb.mode_list) {
+    if (p.kind == PLAN_GRAVITY || !p.nb.mode_list) {
         free_nl(c->nl);
-        NBX_HIP(c, hipMalloc((void**)&c->nl.d_flags, sizeof(int) * 8));
-        NBX_HIP(c, hipMemsetAsync(c->nl.d_flags, 0, sizeof(int) * 8, c->stream));
+        int rc = alloc_flags(c);
+        if (rc) return rc;
         c->nl.ready = true;
         return NBX_OK;
     }
@@ -233,16 +314,31 @@ static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
     return alloc_lists(c, p, grow);
 }
 
-// d_x -> d_pos, list rebuild forced, a = f(x, v) (no step taken)
+// wrapped copy of X[0] into P[0], binned into B[0], list build requested
+static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
+    const int pk = pos_kind(p);
+    if (pk == 0) return;
+    const bool list = pk == 1 && c->nl.ncell >= 3 && bins;
+    if (list) {
+        const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
+        mx_launch_fill_int(c->stream, c->nl.d_cell_count, (int64_t)(2 * ncells), 0);
+    }
+    mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, list ? bin_count(c, 0) : nullptr,
+                        list ? bin_atoms(c, 0) : nullptr,
+                        (pk == 1 && bins) ? req_flag(c, 0) : nullptr);
+}
+
+// a = f(x, v) at the current state (no step taken)
 static int prologue(nbx_ctx* c, mx_plan& p) {
     hipStream_t s = c->stream;
-    mx_launch_positions(s, c, pos_kind(p));
+    refresh_positions(c, p, true);
     if (c->therm == NBX_THERM_BERENDSEN)
         mx_launch_temperature(s, c, c->d_v, constraints(c), c->d_scal);
     // the Berendsen term of the first evaluation uses v itself (vh := v)
     NBX_HIP(c, hipMemcpyAsync(c->d_vh, c->d_v, sizeof(double) * 3 * (size_t)c->n,
                               hipMemcpyDeviceToDevice, s));
-    launch_force(c, p, kick_params(c, 0, 0.0, 0, 0));
+    launch_build(c, p, 0);
+    launch_step(c, p, kick_params(c, 0, 0.0, 0), 0, 0);
     NBX_HIP(c, hipGetLastError());
     c->accel_valid = true;
     return NBX_OK;
@@ -258,18 +354,12 @@ static int grow_lists(nbx_ctx* c, mx_plan& p, double& grow) {
     grow *= 1.6;
     if (grow > 40.0)
         return nbx_fail(c, NBX_ERR_CAPACITY, "neighbour/cell list overflow persists after growth");
-    int rc = prepare(c, p, grow);
-    if (rc) return rc;
-    mx_launch_positions(c->stream, c, pos_kind(p));  // also requests a rebuild
-    NBX_HIP(c, hipGetLastError());
-    return NBX_OK;
+    return prepare(c, p, grow);
 }
 
-int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every, double* uf,
-          double* vf, int64_t* nframes) {
-    mx_plan p;
-    double grow = 1.0;
+static int ensure_ready(nbx_ctx* c, mx_plan& p) {
     int rc;
+    double grow = 1.0;
     if (!c->nl.ready || !c->accel_valid) {
         if ((rc = prepare(c, p, grow))) return rc;
         if ((rc = prologue(c, p))) return rc;
@@ -280,25 +370,73 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
             if ((rc = prologue(c, p))) return rc;
             if ((rc = read_flags(c, h))) return rc;
         }
-    } else {
-        std::string why;
-        if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+        return NBX_OK;
     }
+    std::string why;
+    if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+    return NBX_OK;
+}
+
+int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every, double* uf,
+          double* vf, int64_t* nframes) {
+    mx_plan p;
+    int rc;
+    if ((rc = ensure_ready(c, p))) return rc;
+    double grow = 1.0;
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     backup4 bk;
     NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * 4 * n3));
-    const int pk = pos_kind(p);
-    const double half_skin2 = 0.25 * c->nl.skin * c->nl.skin;
     const int64_t chunk_max = 500;
+    // steps k = 1..count of the chunk, absolute step *d_step + k; then *d_step += count
+    auto enqueue_steps = [&](int64_t k0, int64_t count) {
+        for (int64_t k = k0 + 1; k <= k0 + count; ++k) {
+            const int par = (int)(k & 1);
+            mx_kick_params kp = kick_params(c, 1, dt, seed);
+            kp.step_base = c->d_step;
+            kp.step_off = k - k0;
+            launch_build(c, p, par);
+            launch_step(c, p, kp, par, 1);
+        }
+        mx_launch_step_advance(s, c->d_step, count, 0);
+    };
+    auto graph_for = [&](int gsteps, hipGraphExec_t* out) -> int {
+        nbx_graph_key key;
+        key.valid = 1;
+        key.version = c->version;
+        key.dt = dt;
+        key.seed = seed;
+        key.nbr = c->nl.d_nbr;
+        key.pos = c->d_pos;
+        key.part = c->d_part;
+        key.cap = c->nl.cap;
+        key.ccap = c->nl.ccap;
+        key.ncell = c->nl.ncell;
+        key.lanes = c->nl.lanes;
+        key.steps = gsteps;
+        if (c->gexec && key == c->gkey) { *out = c->gexec; return NBX_OK; }
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        hipGraph_t g = nullptr;
+        NBX_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_steps(0, gsteps);
+        NBX_HIP(c, hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec = nullptr;
+            return nbx_fail(c, NBX_ERR_DEVICE, std::string("hipGraphInstantiate: ") +
+                                                   hipGetErrorString(e));
+        }
+        c->gkey = key;
+        *out = c->gexec;
+        return NBX_OK;
+    };
+    NBX_HIP(c, hipGetLastError());
     int64_t done = 0;
     while (done < nsteps) {
         int64_t chunk = std::min<int64_t>(chunk_max, nsteps - done);
-        if (save_every > 0) {
-            const int64_t to_save = save_every - (done % save_every);
-            chunk = std::min(chunk, to_save);
-        }
+        if (save_every > 0) chunk = std::min(chunk, save_every - (done % save_every));
         // device-side backup of the state for an overflow replay
         const double* src[4] = {c->d_x, c->d_v, c->d_a, c->d_vh};
         for (int k = 0; k < 4; ++k)
@@ -306,16 +444,32 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                                       hipMemcpyDeviceToDevice, s));
         const int64_t step0 = c->steps_taken;
         for (int attempt = 0;; ++attempt) {
-            for (int64_t k = 1; k <= chunk; ++k) {
-                const int64_t step = step0 + k;
-                const mx_kick_params kp = kick_params(c, 1, dt, seed, step);
-                mx_launch_drift(s, c, dt, c->therm, kp.c1, kp.c2, c->therm_T, c->kb, seed, step, pk,
-                                half_skin2);
-                if (c->therm == NBX_THERM_BERENDSEN)
-                    mx_launch_temperature(s, c, c->d_vh, constraints(c), c->d_scal);
-                launch_force(c, p, kp);
+            mx_launch_step_advance(s, c->d_step, step0, 1);
+            // start of step 1 of the chunk: X[0] -> X[1], bins B[1], request req[1]
+            if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3) {
+                const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
+                mx_launch_fill_int(s, bin_count(c, 1), (int64_t)ncells, 0);
             }
+            if (p.kind != PLAN_GRAVITY && p.nb.mode_list) mx_launch_fill_int(s, req_flag(c, 1), 1, 0);
+            {
+                mx_kick_params kp = kick_params(c, 1, dt, seed);
+                kp.step_base = c->d_step;
+                kp.step_off = 0;
+                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1));
+                after_drift(c, p, 1, (int)nbx_blocks(n, 256));
+            }
+            const int gs = c->graph_steps;
+            int64_t k0 = 0;
+            if (gs > 0 && (gs % 2) == 0 && chunk >= gs) {
+                hipGraphExec_t ge = nullptr;
+                if ((rc = graph_for(gs, &ge))) return rc;
+                for (; k0 + gs <= chunk; k0 += gs) NBX_HIP(c, hipGraphLaunch(ge, s));
+            }
+            if (k0 < chunk) enqueue_steps(k0, chunk - k0);
             NBX_HIP(c, hipGetLastError());
+            if (chunk & 1)  // the state is in X[1]
+                NBX_HIP(c, hipMemcpyAsync(c->d_x, c->d_x2, sizeof(double) * n3,
+                                          hipMemcpyDeviceToDevice, s));
             int h[8];
             if ((rc = read_flags(c, h))) return rc;
             if (!h[1]) break;
@@ -325,6 +479,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 NBX_HIP(c, hipMemcpyAsync(dst[k], bk.p + k * n3, sizeof(double) * n3,
                                           hipMemcpyDeviceToDevice, s));
             if ((rc = grow_lists(c, p, grow))) return rc;
+            if ((rc = prologue(c, p))) return rc;
             if (attempt > 8)
                 return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow replay did not converge");
         }
@@ -348,20 +503,16 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
     mx_plan p;
     int rc;
-    if (!c->nl.ready || !c->accel_valid) {
-        if ((rc = prepare(c, p, 1.0))) return rc;
-        if ((rc = prologue(c, p))) return rc;
-    } else {
-        std::string why;
-        if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
-    }
+    if ((rc = ensure_ready(c, p))) return rc;
     hipStream_t s = c->stream;
-    const mx_kick_params kp = kick_params(c, 0, 0.0, 0, 0);
+    refresh_positions(c, p, false);  // P[0] <- X[0] (the list stays valid for it)
+    const mx_kick_params kp = kick_params(c, 0, 0.0, 0);
+    const mx_io io = make_io(c, p, 0, 0);
     mx_nb_launch nb = p.nb;
     nb.lanes = c->nl.lanes;
     auto launch = [&]() {
-        if (p.kind == PLAN_GRAVITY) mx_launch_gravity(s, c, p.G, kp, 1);
-        else mx_launch_nonbonded(s, nb, c, p.pp, kp, nullptr);
+        if (p.kind == PLAN_GRAVITY) mx_launch_gravity(s, c, c->d_pos, p.G, kp, io, 1);
+        else mx_launch_nonbonded(s, nb, c, c->d_pos, p.pp, kp, io, nullptr);
     };
     launch();  // warm
     hipEvent_t e0, e1;
@@ -381,7 +532,10 @@ int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
         else if (p.nb.mode_list) *name = "k_nb_list";
         else *name = "k_nb_allpairs";
     }
-    // the timed launches recomputed a = f(x) in place (force only, no kick): state unchanged
+    // the timed launches overwrote a with the force of the current positions alone
+    // (water: the non-bonded part; Berendsen: with the look-ahead velocities), so the
+    // next run re-evaluates a; x and v are unchanged
+    c->accel_valid = false;
     return NBX_OK;
 }
 
@@ -398,18 +552,18 @@ int mx_count_pairs(nbx_ctx* c, int64_t* pairs) {
     const int prec = c->prec;
     int rc;
     if (!c->nl.ready || !c->accel_valid || prec != NBX_PREC_MIXED) {
-        if ((rc = prepare(c, p, 1.0))) return rc;
         c->prec = NBX_PREC_MIXED;
-        rc = prologue(c, p);
+        rc = ensure_ready(c, p);
         c->prec = prec;
         if (prec != NBX_PREC_MIXED) c->accel_valid = false;  // the FP64 path recomputes a
         if (rc) return rc;
     }
+    refresh_positions(c, p, false);
     unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_scal + 12);
     NBX_HIP(c, hipMemsetAsync(d, 0, sizeof(unsigned long long), c->stream));
     mx_nb_launch nb = p.nb;
     nb.lanes = c->nl.lanes;
-    mx_launch_count_pairs(c->stream, c, nb, p.pp, d);
+    mx_launch_count_pairs(c->stream, c, nb, c->d_pos, p.pp, d);
     unsigned long long h = 0;
     NBX_HIP(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     NBX_HIP(c, hipStreamSynchronize(c->stream));
@@ -418,9 +572,16 @@ int mx_count_pairs(nbx_ctx* c, int64_t* pairs) {
 }
 
 void mx_release(nbx_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    if (c->d_step) (void)hipFree(c->d_step);
+    c->d_step = nullptr;
     free_nl(c->nl);
     if (c->d_pos) (void)hipFree(c->d_pos);
+    if (c->d_pos2) (void)hipFree(c->d_pos2);
+    if (c->d_x2) (void)hipFree(c->d_x2);
     if (c->d_part) (void)hipFree(c->d_part);
-    c->d_pos = nullptr;
+    c->d_pos = c->d_pos2 = nullptr;
+    c->d_x2 = nullptr;
     c->d_part = nullptr;
 }

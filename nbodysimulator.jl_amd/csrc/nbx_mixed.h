@@ -1,5 +1,15 @@
-// nbx_mixed.h -- parameter blocks of the mixed-precision kernels (FP32 pair forces,
-// FP64 state) shared by nbx_nonbonded.hip, nbx_gravity.hip and nbx_mixed.cpp.
+// nbx_mixed.h -- parameter blocks and per-atom device steps of the mixed-precision path
+// (FP32 pair forces, FP64 state) shared by nbx_nonbonded.hip, nbx_gravity.hip and
+// nbx_mixed.cpp.
+//
+// Step k of a run, with x_k in buffer X[k & 1] and its wrapped copy in P[k & 1]:
+//   build(k)  : if req[k & 1]: Verlet list from the bins B[k & 1]; always clears B[(k+1) & 1]
+//   step(k)   : a_k = f(x_k, vh_k); v_k = vh_k + dt a_k / 2 (+ Langevin B half step,
+//               Andersen collisions); then the start of step k+1: Langevin A half step
+//               (registers only), x_{k+1} = x_k + dt v + dt^2 a_k / 2 into X[(k+1) & 1],
+//               vh_{k+1} = v + dt a_k / 2, wrapped copy into P[(k+1) & 1], binning into
+//               B[(k+1) & 1], skin test against the last build -> req[(k+1) & 1], and the
+//               Berendsen temperature of vh_{k+1} (last-block reduction).
 #pragma once
 #include "nbx_internal.h"
 
@@ -14,7 +24,7 @@ struct mx_pair_params {
     double L, invL;   // cubic box side (list mode)
 };
 
-// End-of-step work fused into the force kernels (one thread per atom / molecule)
+// Thermostat / step parameters
 struct mx_kick_params {
     int do_kick;          // 0: force evaluation only (prologue / timing)
     int therm;            // NBX_THERM_*
@@ -26,48 +36,144 @@ struct mx_kick_params {
     double m0;            // first body's mass (Andersen velocity scale, potential systems)
     uint64_t seed;
     int64_t step;         // absolute step number of the step being finished
+    const int64_t* step_base;  // if set, step = *step_base + step_off (graph replays)
+    int64_t step_off;
     const double* d_T;    // Berendsen: temperature of the half-kicked velocities
 };
 
-// velocity update of one atom at the end of a step: v = vh + dt (a/2), then the
-// Langevin B half-step; Andersen is applied by the caller (per body).
-__device__ __forceinline__ void mx_finish_atom(int64_t i, int64_t n, double ax, double ay,
-                                               double az, const double* __restrict__ vh,
-                                               double* __restrict__ v,
-                                               const double* __restrict__ ms,
-                                               const mx_kick_params& kp) {
-    double vx = vh[i] + kp.dt * (0.5 * ax);
-    double vy = vh[n + i] + kp.dt * (0.5 * ay);
-    double vz = vh[2 * n + i] + kp.dt * (0.5 * az);
+// Buffers of one fused step launch
+struct mx_io {
+    const double* x_in;   // x_k (SoA, unwrapped)
+    double* x_out;        // x_{k+1}
+    void* pos_out;        // wrapped copy of x_{k+1}: double4 (+q) or float4 (+m)
+    int pos_kind;         // 0 none, 1 double4 wrapped, 2 float4
+    double* a;            // a_k out
+    double* v;            // v_k out
+    double* vh;           // in: vh_k, out: vh_{k+1}
+    const double* ms;
+    const double* qs;
+    int do_drift;         // 0: finish the step only (no x_{k+1})
+    // Verlet-list bookkeeping (list mode)
+    const double* xref;   // positions at the last build
+    double half_skin2;
+    int* req_out;         // rebuild request for step k+1
+    int* flags;           // [1] overflow
+    int* bin_count;       // bins B[(k+1) & 1]
+    int* bin_atoms;
+    int ccap, nc;
+    double inv_cs, L;
+    // Berendsen temperature of vh_{k+1}
+    int need_T;
+    double* red_part;     // one partial per block
+    unsigned* red_counter;
+    double* T_out;
+    double T_denom;       // kb (3N - Nc)
+};
+
+// resolve the absolute step number inside a kernel (captured graphs are replayed with a
+// device-side step base that the last node of each replay advances)
+__device__ __forceinline__ void mx_resolve_step(mx_kick_params& kp) {
+    if (kp.step_base != nullptr) kp.step = *kp.step_base + kp.step_off;
+}
+
+// v = vh + dt (a/2), then the Langevin B half-step of step kp.step
+__device__ __forceinline__ void mx_kick_atom(int64_t i, int64_t n, double ax, double ay,
+                                             double az, const double* __restrict__ vh,
+                                             const double* __restrict__ ms,
+                                             const mx_kick_params& kp, double* v3) {
+    v3[0] = vh[i] + kp.dt * (0.5 * ax);
+    v3[1] = vh[n + i] + kp.dt * (0.5 * ay);
+    v3[2] = vh[2 * n + i] + kp.dt * (0.5 * az);
     if (kp.therm == NBX_THERM_LANGEVIN) {
-        double sig = kp.c2 * sqrt(kp.kb * kp.T / ms[i]);
+        const double sig = kp.c2 * sqrt(kp.kb * kp.T / ms[i]);
         double g[3];
         nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_LANGEVIN_B, g);
+        for (int k = 0; k < 3; ++k) v3[k] = kp.c1 * v3[k] + sig * g[k];
+    }
+}
+
+// Andersen resampling of one atom (src/nbody_simulation_result.jl:523-540)
+__device__ __forceinline__ void mx_andersen_atom(int64_t i, double m, const mx_kick_params& kp,
+                                                 double* v3) {
+    const double vdev = sqrt(kp.kb * kp.T / m);
+    double g[3];
+    nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_ANDERSEN, g);
+    for (int k = 0; k < 3; ++k) v3[k] = vdev * g[k];
+}
+
+__device__ __forceinline__ int mx_cell_coord(double w, double inv_cs, int nc) {
+    int cx = (int)(w * inv_cs);
+    return cx < 0 ? 0 : (cx >= nc ? nc - 1 : cx);
+}
+
+__device__ __forceinline__ double mx_wrap(double p, double L) {
+    double w = p - L * floor(p / L);
+    return w >= L ? w - L : w;
+}
+
+// Start of step kp.step + 1 for atom i from v (after the previous step's end) and a:
+// Langevin A half-step in registers, drift, half kick, wrapped copy, bins, skin test.
+// Returns m vh^2 (the Berendsen temperature input).
+__device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const double* v3,
+                                                double ax, double ay, double az,
+                                                const mx_kick_params& kp, const mx_io& io) {
+    double vx = v3[0], vy = v3[1], vz = v3[2];
+    const double mi = io.ms[i];
+    if (kp.therm == NBX_THERM_LANGEVIN) {
+        const double sig = kp.c2 * sqrt(kp.kb * kp.T / mi);
+        double g[3];
+        nbx_rng_normals3(kp.seed, i, kp.step + 1, NBX_RNG_LANGEVIN_A, g);
         vx = kp.c1 * vx + sig * g[0];
         vy = kp.c1 * vy + sig * g[1];
         vz = kp.c1 * vz + sig * g[2];
     }
-    v[i] = vx;
-    v[n + i] = vy;
-    v[2 * n + i] = vz;
+    const double dt = kp.dt, dtsq = dt * dt;
+    const double px = io.x_in[i] + dt * vx + dtsq * (0.5 * ax);
+    const double py = io.x_in[n + i] + dt * vy + dtsq * (0.5 * ay);
+    const double pz = io.x_in[2 * n + i] + dt * vz + dtsq * (0.5 * az);
+    io.x_out[i] = px;
+    io.x_out[n + i] = py;
+    io.x_out[2 * n + i] = pz;
+    const double hx = vx + dt * (0.5 * ax), hy = vy + dt * (0.5 * ay), hz = vz + dt * (0.5 * az);
+    io.vh[i] = hx;
+    io.vh[n + i] = hy;
+    io.vh[2 * n + i] = hz;
+    if (io.pos_kind == 1) {
+        const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
+        ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, io.qs != nullptr ? io.qs[i] : 0.0);
+    } else if (io.pos_kind == 2) {
+        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)mi);
+    }
+    if (io.xref != nullptr) {
+        const double ex = px - io.xref[i], ey = py - io.xref[n + i], ez = pz - io.xref[2 * n + i];
+        if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
+    }
+    return mi * (hx * hx + hy * hy + hz * hz);
 }
 
-__device__ __forceinline__ void mx_andersen_atom(int64_t i, int64_t n, double m,
-                                                 double* __restrict__ v,
-                                                 const mx_kick_params& kp) {
-    const double vdev = sqrt(kp.kb * kp.T / m);
-    double g[3];
-    nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_ANDERSEN, g);
-    v[i] = vdev * g[0];
-    v[n + i] = vdev * g[1];
-    v[2 * n + i] = vdev * g[2];
-}
-
-// Berendsen term gamma (T0/T - 1) vh (src/thermostats.jl:76-83)
+// Berendsen coefficient gamma (T0/T - 1) (src/thermostats.jl:76-83)
 __device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
     const double T = kp.d_T[0];
     if (1.0 / T == __builtin_inf()) return kp.ber_gamma;
     return kp.ber_gamma * (kp.T / T - 1.0);
+}
+
+// Epilogue of a step kernel, called by every thread of every block (blockDim.x <= 256):
+// with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x]; the
+// finalize kernel launched next adds the partials in block order (deterministic).  No
+// cross-block fence is needed: the kernel boundary orders the partials before the sum.
+__device__ __forceinline__ void mx_epilogue(double val, const mx_io& io, int n) {
+    (void)n;
+    if (!io.need_T) return;
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    red[t] = val;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off && t + off < (int)blockDim.x) red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0) io.red_part[blockIdx.x] = red[0];
 }
 
 // launchers (nbx_nonbonded.hip / nbx_gravity.hip)
@@ -77,18 +183,34 @@ struct mx_nb_launch {
     int has_lj, el_kind;  // el_kind: 0 none, 1 Coulomb, 2 reaction field
     int water;
 };
+// non-bonded forces at P (list mode) or io.x_in (all pairs); potential systems finish
+// the step (kp.do_kick) and drift (io.do_drift); water stores a_nb only.
 void mx_launch_nonbonded(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c,
-                         const mx_pair_params& pp, const mx_kick_params& kp,
-                         int* clear_flag);
-void mx_launch_drift(hipStream_t s, const nbx_ctx* c, double dt, int therm, double c1,
-                     double c2, double T, double kb, uint64_t seed, int64_t step,
-                     int write_pos_kind, double half_skin2);
-void mx_launch_rebuild(hipStream_t s, const nbx_ctx* c, int water);
-void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp);
+                         const void* pos_in, const mx_pair_params& pp,
+                         const mx_kick_params& kp, const mx_io& io, int* req_clear);
+void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
+                               const mx_io& io);
+// start of step kp.step + 1 from (x_in, v, a): drift of every atom
+void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
+                     const mx_io& io);
+// Verlet list from bins B (gated by req), clearing bins `clear_count` first
+void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
+                     const double* x, const int* req, const int* bin_count,
+                     const int* bin_atoms, int* clear_count);
+// wrapped copy of x into pos, binned into bins; sets *req
+void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
+                         int pos_kind, int* bin_count, int* bin_atoms, int* req);
 void mx_launch_temperature(hipStream_t s, const nbx_ctx* c, const double* v, int64_t Nc,
                            double* d_T);
-void mx_launch_positions(hipStream_t s, const nbx_ctx* c, int write_pos_kind);
 void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& L,
-                           const mx_pair_params& pp, unsigned long long* d_count);
-void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, double G, const mx_kick_params& kp,
-                       int force_only_kernel);
+                           const void* pos, const mx_pair_params& pp,
+                           unsigned long long* d_count);
+void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double G,
+                       const mx_kick_params& kp, const mx_io& io, int force_only_kernel);
+void mx_launch_step_advance(hipStream_t s, int64_t* step_base, int64_t delta, int set);
+void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value);
+// bins of P (gated by *req; the bins were cleared by the preceding build kernel)
+void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* req,
+                   int* bin_count, int* bin_atoms);
+// T = (sum of nparts per-block partials, in order) / denom
+void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* T);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
+    "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
 }
 
 _lib = None
@@ -259,6 +260,12 @@ class Context:
         p = C.c_int64()
         self._check(self._lib.nbx_count_pairs(self._h, C.byref(p)))
         return p.value
+
+    def stats(self) -> dict:
+        out = (C.c_int64 * 5)()
+        self._check(self._lib.nbx_get_stats(self._h, out, 5))
+        return dict(list_builds=out[0], list_slots=out[1], cells_per_side=out[2],
+                    lanes=out[3], graph_steps=out[4])
 
 
 def device_count() -> int:

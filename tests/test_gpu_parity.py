@@ -199,7 +199,10 @@ def test_energies_and_temperature(c1):
     ctx.set_state(c1.u, c1.v)
     ke, pe, pg = ctx.energies()
     assert ke == orc.kinetic_energy(c1.v)
-    assert pe == pytest.approx(orc.potential_energy(c1.u), rel=1e-12)
+    # The reference adds all 372,816 pair terms into one running sum; the GPU sums per
+    # atom, then over atoms.  Both orders carry rounding error of ~1e-12 relative here
+    # (the terms cancel), so the bar is 1e-11; the forces above are bit-identical.
+    assert pe == pytest.approx(orc.potential_energy(c1.u), rel=1e-11)
     assert pg == 0.0
     lib = O.load()
     T = lib.orc_md_temperature(O.dp(c1.v), O.dp(np.ones(864)), 1.0, 864, 0)
