@@ -169,6 +169,43 @@ int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
  * [3] lanes per atom in the force kernel, [4] steps per captured graph.  nout <= 5. */
 int nbx_get_stats(nbx_ctx* ctx, int64_t* out, int32_t nout);
 
+/* ---------------------------------------------------------------------------------
+ * Batched replicas: many independent Lennard-Jones systems (one NBodySimulation each,
+ * src/nbody_simulation.jl:42-54, with PotentialNBodySystem + LennardJonesParameters and
+ * CubicPeriodicBoundaryConditions) integrated together by one persistent launch in
+ * which each replica is owned by one workgroup.  The reference runs such an ensemble
+ * as separate run_simulation calls (src/nbody_simulation_result.jl:468-487); results per
+ * replica follow the single-system mixed path (FP32 pair forces with Kahan sums, FP64
+ * state, velocity Verlet, Andersen / Langevin).  Replica r draws its random numbers from
+ * Philox keyed by seed + r * 0x9E3779B97F4A7C15 with counter (local atom, step), i.e. the
+ * streams of a single-system run of replica r with that seed.
+ * --------------------------------------------------------------------------------- */
+typedef struct nbx_batch nbx_batch;
+
+/* nrep replicas of natoms[r] atoms (<= 4096 each) in cubic boxes of side box_L[r];
+ * masses: sum(natoms) values, replicas concatenated.  One LJ law (eps, sigma, R) for all. */
+int nbx_batch_create(nbx_batch** out, int32_t nrep, const int64_t* natoms,
+                     const double* box_L, const double* masses, double eps, double sigma,
+                     double R, double kb);
+int nbx_batch_destroy(nbx_batch* b);
+const char* nbx_batch_last_error(const nbx_batch* b);
+/* per replica: NBX_THERM_NONE, NBX_THERM_ANDERSEN (param nu) or NBX_THERM_LANGEVIN (gamma) */
+int nbx_batch_set_thermostat(nbx_batch* b, int32_t replica, int32_t kind, double T,
+                             double param);
+int nbx_batch_set_skin(nbx_batch* b, double skin);
+/* u, v: 3 x sum(natoms) column-major, replicas concatenated */
+int nbx_batch_set_state(nbx_batch* b, const double* u, const double* v);
+int nbx_batch_get_state(const nbx_batch* b, double* u, double* v);
+/* nsteps velocity-Verlet steps of every replica; steps_per_launch steps per persistent
+ * launch (0: 1000), the launches replayed from a captured hipGraph. */
+int nbx_batch_run(nbx_batch* b, double dt, int64_t nsteps, uint64_t seed,
+                  int64_t steps_per_launch);
+/* per-replica kinetic and potential energy (reference conventions) and in-cutoff
+ * ordered pair counts of the current state; each array holds nrep values (or NULL) */
+int nbx_batch_energies(nbx_batch* b, double* ke, double* pe, int64_t* pairs);
+/* out[0] list builds summed over replicas, [1] steps taken */
+int nbx_batch_stats(nbx_batch* b, int64_t* out, int32_t nout);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,8 @@ SOURCES = [
     ("nbx_ref64.hip", ["-ffp-contract=off"]),
     ("nbx_nonbonded.hip", []),
     ("nbx_gravity.hip", []),
+    ("nbx_batch.hip", []),
+    ("nbx_batch.cpp", []),
     ("nbx_api.cpp", ["-ffp-contract=off"]),
     ("nbx_mixed.cpp", []),
 ]
@@ -43,7 +45,7 @@ def hipcc() -> str:
 
 def compile_one(src: str, extra: list[str]) -> str:
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(BUILD, src + ".o")
     common = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
               "-I", CSRC]
     if src.endswith(".hip"):

@@ -1,0 +1,206 @@
+"""
+    NBodySimulatorMI355X
+
+`ccall` layer that runs NBodySimulator.jl's force evaluation and velocity-Verlet step on an
+MI355X through libnbx (include/nbx.h).  NBodySimulator's own types (`NBodySimulation`,
+`PotentialNBodySystem`, `LennardJonesParameters`, `ElectrostaticParameters`,
+`GravitationalParameters`, `SPCFwParameters`, `WaterSPCFw`, the boundary conditions and
+thermostats) are used unchanged.  Two drop-in points:
+
+  * `run_simulation(sim, MI355XVerlet(); dt)` replaces the `VelocityVerlet()` solve of
+    `calculate_simulation` (src/nbody_simulation_result.jl:480-487) and returns a
+    `SimulationResult` over an `ODESolution` of `ArrayPartition(v, u)` frames, so
+    `get_position`, `temperature`, energies and `rdf` keep working;
+  * `get_accelerating_function(MI355XPotential(p), sim)` replaces the per-atom closure of
+    `p` (src/nbody_to_ode.jl:156-242): the first call of an RHS evaluation computes all
+    atoms on the GPU, later calls add the cached column.
+
+This file is the reference-side binding; it is not exercised by the C++/Python tests of
+this repository (no Julia toolchain there).
+"""
+module NBodySimulatorMI355X
+
+using NBodySimulator
+using NBodySimulator: NBodySimulation, PotentialNBodySystem, WaterSPCFw, LennardJonesParameters,
+                      ElectrostaticParameters, GravitationalParameters, SPCFwParameters,
+                      CubicPeriodicBoundaryConditions, PeriodicBoundaryConditions, InfiniteBox,
+                      AndersenThermostat, BerendsenThermostat, LangevinThermostat,
+                      NullThermostat, SimulationResult, get_masses
+import NBodySimulator: get_accelerating_function, run_simulation
+import SciMLBase
+using RecursiveArrayTools: ArrayPartition
+
+export MI355XVerlet, MI355XPotential
+
+const libnbx = get(ENV, "NBX_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libnbx.so"))
+
+const NBX_BC_INFINITE, NBX_BC_CUBIC, NBX_BC_PERIODIC = Int32(0), Int32(1), Int32(2)
+const NBX_SYS_POTENTIAL, NBX_SYS_WATER = Int32(0), Int32(1)
+const NBX_PREC_FP64_REF, NBX_PREC_MIXED = Int32(0), Int32(1)
+const NBX_THERM_NONE, NBX_THERM_ANDERSEN, NBX_THERM_BERENDSEN, NBX_THERM_LANGEVIN =
+    Int32(0), Int32(1), Int32(2), Int32(3)
+
+"""
+    MI355XVerlet(; precision = :fp64, seed = 0, skin = nothing)
+
+Velocity Verlet on the GPU.  `precision = :fp64` reproduces the reference arithmetic bit
+for bit; `:mixed` evaluates pair forces in FP32 (Kahan sums) with FP64 state and
+cell/Verlet lists.
+"""
+Base.@kwdef struct MI355XVerlet
+    precision::Symbol = :fp64
+    seed::UInt64 = 0
+    skin::Union{Nothing, Float64} = nothing
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    n::Int
+end
+
+function check(ctx::Context, rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:nbx_last_error, libnbx), Cstring, (Ptr{Cvoid},), ctx.ptr))
+    rc == -2 && throw(BoundsError(msg))
+    rc == -1 && throw(ArgumentError(msg))
+    error("libnbx error $rc: $msg")
+end
+
+function Context(n::Integer, kind::Int32, ms::Vector{Float64}, qs::Union{Nothing, Vector{Float64}})
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:nbx_create, libnbx), Cint,
+               (Ref{Ptr{Cvoid}}, Int64, Int32, Ptr{Float64}, Ptr{Float64}),
+               out, n, kind, ms, qs === nothing ? C_NULL : qs)
+    if rc != 0
+        msg = unsafe_string(ccall((:nbx_last_error, libnbx), Cstring, (Ptr{Cvoid},), C_NULL))
+        error("nbx_create failed ($rc): $msg")
+    end
+    ctx = Context(out[], Int(n))
+    finalizer(c -> ccall((:nbx_destroy, libnbx), Cint, (Ptr{Cvoid},), c.ptr), ctx)
+    return ctx
+end
+
+set_boundary!(ctx, bc::InfiniteBox) =
+    check(ctx, ccall((:nbx_set_boundary, libnbx), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                     ctx.ptr, NBX_BC_INFINITE, C_NULL))
+set_boundary!(ctx, bc::CubicPeriodicBoundaryConditions) =
+    check(ctx, ccall((:nbx_set_boundary, libnbx), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                     ctx.ptr, NBX_BC_CUBIC, Float64[bc.L, 0, 0, 0, 0, 0]))
+set_boundary!(ctx, bc::PeriodicBoundaryConditions) =
+    check(ctx, ccall((:nbx_set_boundary, libnbx), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                     ctx.ptr, NBX_BC_PERIODIC, Float64.(collect(bc.boundary))))
+
+add_potential!(ctx, p::LennardJonesParameters) =
+    check(ctx, ccall((:nbx_add_lennard_jones, libnbx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble),
+                     ctx.ptr, p.ϵ, p.σ, p.R))
+add_potential!(ctx, p::ElectrostaticParameters) =
+    check(ctx, ccall((:nbx_add_electrostatic, libnbx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble),
+                     ctx.ptr, p.k, p.R))
+add_potential!(ctx, p::GravitationalParameters) =
+    check(ctx, ccall((:nbx_add_gravitational, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, p.G))
+add_potential!(ctx, p) = throw(ArgumentError("no GPU kernel for $(typeof(p))"))
+
+function set_thermostat!(ctx, th)
+    kind, T, par = th isa AndersenThermostat ? (NBX_THERM_ANDERSEN, th.T, th.ν) :
+                   th isa BerendsenThermostat ? (NBX_THERM_BERENDSEN, th.T, th.τ) :
+                   th isa LangevinThermostat ? (NBX_THERM_LANGEVIN, th.T, th.γ) :
+                   th isa NullThermostat ? (NBX_THERM_NONE, 0.0, 1.0) :
+                   throw(ArgumentError("thermostat $(typeof(th)) has no GPU kernel"))
+    check(ctx, ccall((:nbx_set_thermostat, libnbx), Cint, (Ptr{Cvoid}, Int32, Cdouble, Cdouble),
+                     ctx.ptr, kind, T, par))
+end
+
+charges(s::PotentialNBodySystem) =
+    hasproperty(first(s.bodies), :q) ? Float64[b.q for b in s.bodies] : nothing
+function charges(s::WaterSPCFw)
+    q = zeros(3 * length(s.bodies))
+    q[1:3:end] .= s.qO; q[2:3:end] .= s.qH; q[3:3:end] .= s.qH
+    return q
+end
+
+"""Context holding the simulation's potentials, boundary, kb and thermostat."""
+function build_context(sim::NBodySimulation, precision::Symbol; only = nothing)
+    sys = sim.system
+    ms = Float64.(get_masses(sys))
+    if sys isa WaterSPCFw
+        ctx = Context(length(ms), NBX_SYS_WATER, ms, charges(sys))
+        set_boundary!(ctx, sim.boundary_conditions)
+        add_potential!(ctx, sys.lj_parameters)
+        add_potential!(ctx, sys.e_parameters)
+        p = sys.scpfw_parameters
+        check(ctx, ccall((:nbx_set_spcfw, libnbx), Cint,
+                         (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble),
+                         ctx.ptr, p.rOH, p.aHOH, p.kb, p.ka))
+    else
+        ctx = Context(length(ms), NBX_SYS_POTENTIAL, ms, charges(sys))
+        set_boundary!(ctx, sim.boundary_conditions)
+        for p in (only === nothing ? values(sys.potentials) : (only,))
+            add_potential!(ctx, p)
+        end
+    end
+    check(ctx, ccall((:nbx_set_kb, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, sim.kb))
+    check(ctx, ccall((:nbx_set_precision, libnbx), Cint, (Ptr{Cvoid}, Int32), ctx.ptr,
+                     precision === :mixed ? NBX_PREC_MIXED : NBX_PREC_FP64_REF))
+    only === nothing && set_thermostat!(ctx, sim.thermostat)
+    return ctx
+end
+
+"""
+    run_simulation(sim, alg::MI355XVerlet; dt, save_everystep = true)
+
+GPU replacement of `calculate_simulation(sim, VelocityVerlet(); dt)`.
+"""
+function run_simulation(sim::NBodySimulation, alg::MI355XVerlet; dt::Real,
+                        save_everystep::Bool = true)
+    ctx = build_context(sim, alg.precision)
+    alg.skin === nothing ||
+        check(ctx, ccall((:nbx_set_skin, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, alg.skin))
+    u0, v0, _ = NBodySimulator.gather_bodies_initial_coordinates(sim)
+    u0 = Matrix{Float64}(u0); v0 = Matrix{Float64}(v0)
+    n = ctx.n
+    check(ctx, ccall((:nbx_set_state, libnbx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.ptr, u0, v0))
+    t1, t2 = sim.tspan
+    nsteps = round(Int, (t2 - t1) / dt)
+    every = save_everystep ? 1 : max(nsteps, 1)
+    nframes = 1 + nsteps ÷ every
+    uf = Array{Float64}(undef, 3, n, nframes)
+    vf = Array{Float64}(undef, 3, n, nframes)
+    check(ctx, ccall((:nbx_vv_run, libnbx), Cint,
+                     (Ptr{Cvoid}, Cdouble, Int64, UInt64, Int64, Ptr{Float64}, Ptr{Float64}),
+                     ctx.ptr, dt, nsteps, alg.seed, every, uf, vf))
+    ts = [t1 + k * every * dt for k in 0:(nframes - 1)]
+    us = [ArrayPartition(vf[:, :, k], uf[:, :, k]) for k in 1:nframes]
+    prob = SciMLBase.SecondOrderODEProblem(sim)
+    sol = SciMLBase.build_solution(prob, alg, ts, us)
+    sol.stats.naccept = nsteps
+    return SimulationResult(sol, sim)
+end
+
+"""Per-atom drop-in for the closure of a potential parameter object."""
+struct MI355XPotential{P}
+    p::P
+end
+
+function get_accelerating_function(w::MI355XPotential, sim::NBodySimulation)
+    ctx = build_context(sim, :fp64; only = w.p)
+    n = ctx.n
+    cache = zeros(3, n)
+    last_u = fill(NaN, 3, n)
+    return (dv, u, v, t, i) -> begin
+        if u != last_u
+            fill!(cache, 0.0)
+            uu = Matrix{Float64}(u)
+            check(ctx, ccall((:nbx_accelerations, libnbx), Cint,
+                             (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Int64, Ptr{Float64}),
+                             ctx.ptr, 0, uu, 0, n, cache))
+            last_u .= u
+        end
+        @inbounds for k in 1:3
+            dv[k] += cache[k, i]
+        end
+        dv
+    end
+end
+
+end # module

@@ -63,6 +63,17 @@ SIGNATURES = {
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
     "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
+    "nbx_batch_create": (C.c_int, [C.POINTER(_P), _I32, C.POINTER(_I64), _DP, _DP, _D, _D,
+                                   _D, _D]),
+    "nbx_batch_destroy": (C.c_int, [_P]),
+    "nbx_batch_last_error": (C.c_char_p, [_P]),
+    "nbx_batch_set_thermostat": (C.c_int, [_P, _I32, _I32, _D, _D]),
+    "nbx_batch_set_skin": (C.c_int, [_P, _D]),
+    "nbx_batch_set_state": (C.c_int, [_P, _DP, _DP]),
+    "nbx_batch_get_state": (C.c_int, [_P, _DP, _DP]),
+    "nbx_batch_run": (C.c_int, [_P, _D, _I64, _U64, _I64]),
+    "nbx_batch_energies": (C.c_int, [_P, _DP, _DP, C.POINTER(_I64)]),
+    "nbx_batch_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
 }
 
 _lib = None
@@ -266,6 +277,76 @@ class Context:
         self._check(self._lib.nbx_get_stats(self._h, out, 5))
         return dict(list_builds=out[0], list_slots=out[1], cells_per_side=out[2],
                     lanes=out[3], graph_steps=out[4])
+
+
+class Batch:
+    """Owns one nbx_batch: independent LJ replicas integrated by one persistent launch."""
+
+    def __init__(self, natoms, box_L, masses, eps=1.0, sigma=1.0, R=2.5, kb=1.0):
+        lib = load()
+        self._lib = lib
+        self.natoms = np.ascontiguousarray(np.asarray(natoms, dtype=np.int64))
+        self.nrep = len(self.natoms)
+        self.ntot = int(self.natoms.sum())
+        box = np.ascontiguousarray(np.asarray(box_L, dtype=np.float64))
+        ms = np.ascontiguousarray(np.asarray(masses, dtype=np.float64))
+        h = _P()
+        rc = lib.nbx_batch_create(C.byref(h), self.nrep,
+                                  self.natoms.ctypes.data_as(C.POINTER(_I64)), dptr(box),
+                                  dptr(ms), float(eps), float(sigma), float(R), float(kb))
+        if rc != NBX_OK:
+            raise_for(rc, lib.nbx_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != NBX_OK:
+            raise_for(rc, self._lib.nbx_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_thermostat(self, replica, kind, T=0.0, param=1.0):
+        self._check(self._lib.nbx_batch_set_thermostat(self._h, int(replica), int(kind),
+                                                       float(T), float(param)))
+
+    def set_skin(self, skin):
+        self._check(self._lib.nbx_batch_set_skin(self._h, float(skin)))
+
+    def set_state(self, u, v=None):
+        u = as_c3n(u, self.ntot)
+        v = None if v is None else as_c3n(v, self.ntot)
+        self._check(self._lib.nbx_batch_set_state(self._h, dptr(u), dptr(v)))
+
+    def get_state(self):
+        u = np.empty((self.ntot, 3))
+        v = np.empty((self.ntot, 3))
+        self._check(self._lib.nbx_batch_get_state(self._h, dptr(u), dptr(v)))
+        return u, v
+
+    def run(self, dt, nsteps, seed=0, steps_per_launch=0):
+        self._check(self._lib.nbx_batch_run(self._h, float(dt), int(nsteps), int(seed),
+                                            int(steps_per_launch)))
+
+    def energies(self):
+        ke = np.empty(self.nrep)
+        pe = np.empty(self.nrep)
+        pairs = np.empty(self.nrep, dtype=np.int64)
+        self._check(self._lib.nbx_batch_energies(self._h, dptr(ke), dptr(pe),
+                                                 pairs.ctypes.data_as(C.POINTER(_I64))))
+        return ke, pe, pairs
+
+    def stats(self):
+        out = (C.c_int64 * 2)()
+        self._check(self._lib.nbx_batch_stats(self._h, out, 2))
+        return dict(list_builds=out[0], steps_taken=out[1])
 
 
 def device_count() -> int:

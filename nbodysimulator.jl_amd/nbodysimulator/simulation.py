@@ -233,6 +233,69 @@ def run_simulation(s: NBodySimulation, alg=None, *, dt: float = None,
     return SimulationResult(sol, s, ctx)
 
 
+def run_simulations(sims, alg=None, *, dt: float, seed: int = None,
+                    steps_per_launch: int = 0):
+    """Integrate many independent Lennard-Jones simulations together in one persistent
+    GPU launch (one workgroup per replica).  Each simulation must hold a
+    PotentialNBodySystem with the same LennardJonesParameters, a
+    CubicPeriodicBoundaryConditions box, the same kb and tspan length, and a Null,
+    Andersen or Langevin thermostat.  Returns one SimulationResult per simulation with
+    the initial and final states (the reference runs each with run_simulation)."""
+    if alg is None:
+        alg = VelocityVerlet(precision="mixed")
+    sims = list(sims)
+    if not sims:
+        return []
+    first = sims[0]
+    lj = None
+    natoms, boxes, masses, us, vs = [], [], [], [], []
+    for s in sims:
+        sys_ = s.system
+        if not isinstance(sys_, PotentialNBodySystem) or set(sys_.potentials) != {"lennard_jones"}:
+            raise NotImplementedError("batched runs take Lennard-Jones potential systems")
+        p = sys_.potentials["lennard_jones"]
+        if lj is None:
+            lj = p
+        elif (p.ϵ, p.σ, p.R) != (lj.ϵ, lj.σ, lj.R):
+            raise ValueError("every replica must use the same LennardJonesParameters")
+        if not isinstance(s.boundary_conditions, CubicPeriodicBoundaryConditions):
+            raise NotImplementedError("batched runs need CubicPeriodicBoundaryConditions")
+        if s.kb != first.kb or (s.tspan[1] - s.tspan[0]) != (first.tspan[1] - first.tspan[0]):
+            raise ValueError("replicas must share kb and the length of tspan")
+        u0, v0, n = gather_bodies_initial_coordinates(s)
+        natoms.append(n)
+        boxes.append(float(s.boundary_conditions.L))
+        masses.append(get_masses(sys_))
+        us.append(u0)
+        vs.append(v0)
+    b = _lib.Batch(natoms, boxes, np.concatenate(masses), lj.ϵ, lj.σ, lj.R, first.kb)
+    for r, s in enumerate(sims):
+        th = s.thermostat
+        if isinstance(th, AndersenThermostat):
+            b.set_thermostat(r, _lib.THERM_ANDERSEN, th.T, th.ν)
+        elif isinstance(th, LangevinThermostat):
+            b.set_thermostat(r, _lib.THERM_LANGEVIN, th.T, th.γ)
+        elif not (th is None or isinstance(th, NullThermostat)):
+            raise NotImplementedError(f"batched runs take Andersen or Langevin, not {type(th).__name__}")
+    if alg.skin is not None:
+        b.set_skin(alg.skin)
+    U0, V0 = np.concatenate(us), np.concatenate(vs)
+    b.set_state(U0, V0)
+    t1, t2 = float(first.tspan[0]), float(first.tspan[1])
+    nsteps, dt, last = _step_plan(t1, t2, float(dt))
+    if last:
+        raise ValueError("batched runs need tspan to be a whole number of steps")
+    b.run(dt, nsteps, alg.seed if seed is None else seed, steps_per_launch)
+    U, V = b.get_state()
+    out, o = [], 0
+    for s, n in zip(sims, natoms):
+        sol = Solution(np.array([t1, t2]), np.stack([U0[o:o + n], U[o:o + n]]),
+                       np.stack([V0[o:o + n], V[o:o + n]]), _Stats(nsteps))
+        out.append(SimulationResult(sol, s))
+        o += n
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Accessors and analysis: src/nbody_simulation_result.jl:49-446
 # ---------------------------------------------------------------------------
