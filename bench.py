@@ -194,12 +194,33 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
     return out
 
 
+def profile_kernels(lib, gen):
+    """Short launch sequence for hardware-counter runs: the gravity force kernel at the
+    headline size and a plain copy kernel of known size (the byte-counter calibration)."""
+    n = 65536
+    u, v, m = gen.plummer(n, seed=2024)
+    ctx = lib.Context(n, lib.SYS_POTENTIAL, m)
+    ctx.add_gravitational(1.0)
+    ctx.set_precision(lib.PREC_MIXED)
+    ctx.set_state(u, v)
+    kms, kname = ctx.time_force_kernel(10)
+    nbytes = 1 << 30
+    cms = ctx.copy_probe(nbytes, 5)
+    print(json.dumps({"kernel": kname, "kernel_avg_ms": kms, "copy_bytes": nbytes,
+                      "copy_avg_ms": cms, "copy_GBps": 2 * nbytes / (cms * 1e-3) / 1e9}),
+          flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the dominant kernels (gravity N=65536 force, 10 launches; "
+                         "a 1 GiB copy-kernel calibration, 5 launches) for rocprofv3 counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
@@ -211,6 +232,8 @@ def main():
     if nb.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.profile_kernels:
+        return profile_kernels(lib, gen)
     log("[bench] gravity N=65536")
     g = bench_gravity(args, lib, gen)
     res = {

@@ -164,6 +164,11 @@ int nbx_time_force_kernel(nbx_ctx* ctx, int32_t iters, double* avg_ms,
 /* Ordered in-cutoff pairs of the current state (pair-interaction count of one force
  * evaluation; n(n-1) for gravity). */
 int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
+/* Measurement support (no reference counterpart): average duration (ms) of a plain
+ * device-to-device copy kernel moving `bytes` (rounded down to 16 B) over `iters`
+ * launches on the context's stream.  Calibrates HBM byte counters (the kernel reads
+ * and writes exactly `bytes` each) and gives the achievable bandwidth. */
+int nbx_copy_probe(nbx_ctx* ctx, int64_t bytes, int32_t iters, double* avg_ms);
 /* Diagnostics of the mixed path (no reference counterpart): out[0] Verlet-list builds
  * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),
  * [3] lanes per atom in the force kernel, [4] steps per captured graph.  nout <= 5. */
@@ -205,6 +210,86 @@ int nbx_batch_run(nbx_batch* b, double dt, int64_t nsteps, uint64_t seed,
 int nbx_batch_energies(nbx_batch* b, double* ke, double* pe, int64_t* pairs);
 /* out[0] list builds summed over replicas, [1] steps taken */
 int nbx_batch_stats(nbx_batch* b, int64_t* out, int32_t nout);
+
+/* ---------------------------------------------------------------------------------
+ * Learned stride (no counterpart in the reference; it accelerates the
+ * `solve(SecondOrderODEProblem(sim), VelocityVerlet(); dt)` path of
+ * src/nbody_simulation_result.jl:480-487 for a Lennard-Jones fluid in a cubic box).
+ *
+ * One strided step of length D = stride * dt is a velocity-Verlet step of length D whose
+ * drift and kick carry per-atom corrections predicted by a small MLP:
+ *     x' = x + D v + D^2 a/2 + dx,   a' = f(x'),   v' = v + D (a + a')/2 + dv
+ * (dx, dv with their mean over atoms removed).  Network input per atom (35): 32 Gaussian
+ * radial density bins (centres 0.8 .. 2.5 sigma, standard deviation 0.1 sigma, cosine
+ * switch over 2.0 .. 2.5 sigma), then v.e1, |v - (v.e1) e1| and |a|, where the force
+ * frame is e1 = a/|a|, e2 = the unit part of v normal to e1, e3 = e1 x e2.  Output (6):
+ * dx and dv along (e1, e2, e3), each divided by its output scale.  Layers 35-128-128-
+ * 128-6 with SiLU, BF16 weights and activations, FP32 accumulation (MFMA), FP32 master
+ * weights trained with Adam.  Training targets come from plain velocity-Verlet
+ * trajectories at dt: dx = x(t+D) - (x + D v + D^2 a/2), dv = v(t+D) - (v + D (a +
+ * a(t+D))/2), in the frame at t.
+ * ------------------------------------------------------------------------------- */
+typedef struct nbx_model nbx_model;
+#define NBX_MLP_INPUTS 35
+#define NBX_MLP_OUTPUTS 6
+#define NBX_MLP_PARAMS 38406
+/* normalization block: feature mean[35], feature inverse scale[35], output scale[6] */
+#define NBX_MLP_NORM 76
+
+/* parameters initialised U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from seed; identity
+ * normalization; empty training set */
+int nbx_model_create(nbx_model** out, uint64_t seed);
+int nbx_model_destroy(nbx_model* model);
+const char* nbx_model_last_error(const nbx_model* model);
+/* flat FP32 parameters (W1[128][35], b1, W2[128][128], b2, W3[128][128], b3, W4[6][128],
+ * b4) as doubles, and the normalization block; either pointer may be NULL.  Setting
+ * parameters resets the Adam state. */
+int nbx_model_set(nbx_model* model, const double* params, const double* norm);
+int nbx_model_get(nbx_model* model, double* params, double* norm);
+/* network outputs (normalized units) of n raw feature rows: feats n x 35, out n x 6 */
+int nbx_model_forward(nbx_model* model, const double* feats, int64_t n, double* out);
+/* gradient of mean((out - targets / output_scale)^2) over n x 6 with respect to the flat
+ * parameters, and that loss; n a multiple of 256; targets n x 6 in raw units */
+int nbx_model_gradient(nbx_model* model, const double* feats, const double* targets,
+                       int64_t n, double* grad, double* loss);
+/* training set (device resident): append rows, read rows back, clear */
+int nbx_model_add_samples(nbx_model* model, const double* feats, const double* targets,
+                          int64_t n);
+int nbx_model_get_samples(nbx_model* model, int64_t first, int64_t n, double* feats,
+                          double* targets, int64_t* count);
+int nbx_model_clear_samples(nbx_model* model);
+/* normalization fitted to the training set: feature mean / (1 / standard deviation),
+ * per-component RMS of the targets */
+int nbx_model_fit_normalization(nbx_model* model);
+/* iters Adam steps (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch` rows (a
+ * multiple of 256) drawn with replacement (Philox, seed); losses[iters] or NULL */
+int nbx_model_train(nbx_model* model, int64_t iters, int32_t batch, double lr, uint64_t seed,
+                    double* losses);
+/* per-component mean squared error of the outputs on rows [first, first + n) of the
+ * training set, and (base6, or NULL) that of predicting zero, in normalized units */
+int nbx_model_evaluate(nbx_model* model, int64_t first, int64_t n, double* mse6,
+                       double* base6);
+
+/* descriptor of the context's current state: feats n x 35 (raw), frames n x 6 (e1, e2);
+ * mixed precision, one Lennard-Jones term, cubic box, no thermostat */
+int nbx_features(nbx_ctx* ctx, double* feats, double* frames);
+/* kinetic energy and the Lennard-Jones energy shifted to zero at the cutoff, from the
+ * Verlet list (the reference's potential_energy equals pe_shifted + n(n-1)/2 * U(R),
+ * src/nbody_simulation_result.jl:293-319) */
+int nbx_learn_energy(nbx_ctx* ctx, double* ke, double* pe_shifted);
+/* training set from plain velocity Verlet at dt: per frame, `atoms_per_frame` atoms
+ * drawn with Philox(seed) are described at t, the context advances `stride` steps, and
+ * their targets are appended to the model's training set; nframes frames */
+int nbx_learn_collect(nbx_ctx* ctx, nbx_model* model, double dt, int32_t stride,
+                      int64_t nframes, int32_t atoms_per_frame, uint64_t seed);
+/* nsteps strided steps.  Every check_every strided steps the total (shifted) energy is
+ * compared with the run's initial value; when |E - E0| / |E0| exceeds drift_tol the
+ * state returns to the start of that block and plain velocity Verlet at dt runs the
+ * remaining steps.  report[8]: strided steps taken, plain steps taken, tripped (0/1),
+ * strided steps before the trip (-1: none), E0, final E, largest relative deviation,
+ * Verlet-list builds. */
+int nbx_learned_run(nbx_ctx* ctx, const nbx_model* model, double dt, int32_t stride,
+                    int64_t nsteps, double drift_tol, int32_t check_every, double* report);
 
 #ifdef __cplusplus
 }

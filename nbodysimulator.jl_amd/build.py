@@ -33,6 +33,8 @@ SOURCES = [
     ("nbx_batch.cpp", []),
     ("nbx_api.cpp", ["-ffp-contract=off"]),
     ("nbx_mixed.cpp", []),
+    ("nbx_learn.hip", []),
+    ("nbx_learn.cpp", []),
 ]
 
 

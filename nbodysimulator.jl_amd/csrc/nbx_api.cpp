@@ -671,6 +671,61 @@ extern "C" int nbx_time_force_kernel(nbx_ctx* c, int32_t iters, double* avg_ms,
     return mx_time_force(c, iters, avg_ms, name);
 }
 
+extern "C" int nbx_copy_probe(nbx_ctx* c, int64_t bytes, int32_t iters, double* avg_ms) {
+    if (!c || !avg_ms) return NBX_ERR_INVALID;
+    if (bytes < 16 || iters <= 0) return nbx_fail(c, NBX_ERR_INVALID, "bytes >= 16 and iters > 0 required");
+    return mx_copy_probe(c, bytes, iters, avg_ms);
+}
+
+static int learn_ready(nbx_ctx* c) {
+    if (!c->state_set) return nbx_fail(c, NBX_ERR_INVALID, "no state set");
+    if (c->prec != NBX_PREC_MIXED)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "the learned stride runs on the mixed-precision path");
+    return NBX_OK;
+}
+
+extern "C" int nbx_features(nbx_ctx* c, double* feats, double* frames) {
+    if (!c) return NBX_ERR_INVALID;
+    int rc;
+    if ((rc = learn_ready(c))) return rc;
+    return mx_features(c, feats, frames);
+}
+
+extern "C" int nbx_learn_energy(nbx_ctx* c, double* ke, double* pe) {
+    if (!c || !ke || !pe) return NBX_ERR_INVALID;
+    int rc;
+    if ((rc = learn_ready(c))) return rc;
+    return mx_learn_energy(c, ke, pe);
+}
+
+extern "C" int nbx_learn_collect(nbx_ctx* c, nbx_model* m, double dt, int32_t stride,
+                                 int64_t nframes, int32_t atoms_per_frame, uint64_t seed) {
+    if (!c || !m) return NBX_ERR_INVALID;
+    int rc;
+    if ((rc = learn_ready(c))) return rc;
+    if (!(dt > 0.0) || !std::isfinite(dt)) return nbx_fail(c, NBX_ERR_INVALID, "dt must be > 0");
+    if (stride < 1 || stride > 64) return nbx_fail(c, NBX_ERR_INVALID, "stride must be in 1..64");
+    if (nframes < 0) return nbx_fail(c, NBX_ERR_INVALID, "nframes must be >= 0");
+    if (atoms_per_frame < 1 || atoms_per_frame > c->n)
+        return nbx_fail(c, NBX_ERR_INVALID, "atoms_per_frame must be in 1..n");
+    if (nframes == 0) return NBX_OK;
+    return mx_learn_collect(c, m, dt, stride, nframes, atoms_per_frame, seed);
+}
+
+extern "C" int nbx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int32_t stride,
+                               int64_t nsteps, double drift_tol, int32_t check_every,
+                               double* report) {
+    if (!c || !m) return NBX_ERR_INVALID;
+    int rc;
+    if ((rc = learn_ready(c))) return rc;
+    if (!(dt > 0.0) || !std::isfinite(dt)) return nbx_fail(c, NBX_ERR_INVALID, "dt must be > 0");
+    if (stride < 1 || stride > 64) return nbx_fail(c, NBX_ERR_INVALID, "stride must be in 1..64");
+    if (nsteps < 0) return nbx_fail(c, NBX_ERR_INVALID, "nsteps must be >= 0");
+    if (!(drift_tol > 0.0)) return nbx_fail(c, NBX_ERR_INVALID, "drift_tol must be > 0");
+    if (check_every < 1) return nbx_fail(c, NBX_ERR_INVALID, "check_every must be >= 1");
+    return mx_learned_run(c, m, dt, stride, nsteps, drift_tol, check_every, report);
+}
+
 extern "C" int nbx_get_stats(nbx_ctx* c, int64_t* out, int32_t nout) {
     if (!c || !out || nout < 0 || nout > 5) return NBX_ERR_INVALID;
     int64_t v[5] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps};

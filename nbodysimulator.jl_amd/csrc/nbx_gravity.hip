@@ -136,3 +136,47 @@ int mx_gravity_chunks(int64_t n) {
     while (tblocks * chunks < 2048 && (n / (chunks * 2)) >= 2048) chunks *= 2;
     return chunks;
 }
+
+// ---------------------------------------------------------------------------
+// Plain device-to-device copy: the calibration kernel for HBM byte counters and the
+// achievable-bandwidth probe.  Reads `n16` 16-byte words from src, writes them to dst.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src,
+                                                    float4* __restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms) {
+    const int64_t n16 = bytes / 16;
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess) return nbx_fail(c, NBX_ERR_DEVICE, "copy probe: out of memory");
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return nbx_fail(c, NBX_ERR_DEVICE, "copy probe: out of memory");
+    }
+    hipStream_t s = c->stream;
+    (void)hipMemsetAsync(a, 0, n16 * 16, s);
+    const int blocks = 8192;
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, s, (const float4*)a, (float4*)b, n16);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < iters; ++k)
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, s, (const float4*)a, (float4*)b,
+                           n16);
+    (void)hipEventRecord(e1, s);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return nbx_fail(c, NBX_ERR_DEVICE, hipGetErrorString(err));
+    *avg_ms = (double)ms / iters;
+    return NBX_OK;
+}

@@ -169,5 +169,13 @@ int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d
 
 // ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name);
+int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms);
+// learned stride (nbx_mixed.cpp)
+int mx_features(nbx_ctx* c, double* feats, double* frames);
+int mx_learn_energy(nbx_ctx* c, double* ke, double* pe);
+int mx_learn_collect(nbx_ctx* c, nbx_model* m, double dt, int stride, int64_t nframes, int nsel,
+                     uint64_t seed);
+int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_t nbig,
+                   double tol, int check_every, double* report);
 int mx_count_pairs(nbx_ctx* c, int64_t* pairs);
 void mx_release(nbx_ctx* c);

@@ -585,3 +585,338 @@ void mx_release(nbx_ctx* c) {
     c->d_x2 = nullptr;
     c->d_part = nullptr;
 }
+
+// =================================================================================
+// Learned stride (no counterpart in the reference): descriptor of the current state,
+// training-set collection from plain velocity Verlet, the corrected strided integrator
+// with its energy-drift monitor and fallback, and the list-based energy it monitors.
+// =================================================================================
+#include "nbx_learn_ctx.h"
+#include "nbx_model.h"
+
+namespace {
+
+struct lj_consts {
+    double eps = 0, sigma = 0, R = 0;
+};
+
+int learn_plan(nbx_ctx* c, mx_plan& p, lj_consts& lj) {
+    std::string why;
+    if (c->prec != NBX_PREC_MIXED)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "the learned stride runs on the mixed-precision path");
+    if (c->sys != NBX_SYS_POTENTIAL || c->therm != NBX_THERM_NONE)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "the learned stride needs a potential system without thermostat");
+    if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+    if (p.kind != PLAN_NONBONDED || !p.nb.mode_list || !p.nb.has_lj || p.nb.el_kind != 0)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED,
+                        "the learned stride needs one Lennard-Jones term in a cubic periodic box");
+    for (const nbx_term& t : c->terms)
+        if (t.kind == NBX_TERM_LJ) { lj.eps = t.p[0]; lj.sigma = t.p[1]; lj.R = t.p[2]; }
+    return NBX_OK;
+}
+
+// P[0] <- wrap(X[0]) and a Verlet list built for it (the force a stays as it is)
+int fresh_list(nbx_ctx* c, mx_plan& p) {
+    int rc;
+    double grow = 1.0;
+    if ((rc = ensure_ready(c, p))) return rc;
+    for (int attempt = 0;; ++attempt) {
+        refresh_positions(c, p, true);
+        launch_build(c, p, 0);
+        NBX_HIP(c, hipGetLastError());
+        int h[8];
+        if ((rc = read_flags(c, h))) return rc;
+        if (!h[1]) return NBX_OK;
+        if (attempt > 8) return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow persists");
+        if ((rc = grow_lists(c, p, grow))) return rc;
+        if ((rc = prologue(c, p))) return rc;
+    }
+}
+
+lr_feat_params feat_params(nbx_ctx* c, int par) {
+    lr_feat_params fp{};
+    fp.pos = (const double4*)pbuf(c, par);
+    fp.nbr = c->nl.d_nbr;
+    fp.nnb = c->nl.d_nnb;
+    fp.n = (int)c->n;
+    fp.L = c->bc.b[0];
+    fp.invL = 1.0 / c->bc.b[0];
+    fp.a = c->d_a;
+    fp.v = c->d_v;
+    fp.x = c->d_x;
+    return fp;
+}
+
+lr_energy_params energy_params(nbx_ctx* c, int par, const lj_consts& lj, double* part) {
+    lr_energy_params ep{};
+    ep.pos = (const double4*)pbuf(c, par);
+    ep.nbr = c->nl.d_nbr;
+    ep.nnb = c->nl.d_nnb;
+    ep.n = (int)c->n;
+    ep.L = c->bc.b[0];
+    ep.invL = 1.0 / c->bc.b[0];
+    ep.s2 = lj.sigma * lj.sigma;
+    ep.R2 = lj.R * lj.R;
+    const double sr6 = std::pow(ep.s2 / ep.R2, 3);
+    ep.uc = sr6 * sr6 - sr6;
+    ep.c4eps = 4.0 * lj.eps;
+    ep.v = c->d_v;
+    ep.ms = c->d_ms;
+    ep.part = part;
+    return ep;
+}
+
+struct dev_buf {
+    void* p = nullptr;
+    ~dev_buf() { if (p) (void)hipFree(p); }
+    template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+int dalloc(nbx_ctx* c, dev_buf& b, size_t bytes) {
+    NBX_HIP(c, hipMalloc(&b.p, std::max<size_t>(bytes, 16)));
+    return NBX_OK;
+}
+
+// kinetic and shifted Lennard-Jones energy of the state whose wrapped copy is P[par]
+int list_energy(nbx_ctx* c, int par, const lj_consts& lj, dev_buf& part, double* ke, double* pe) {
+    double* out = c->d_scal + 13;  // [13], [14]
+    lr_launch_energy(c->stream, c->nl.lanes, energy_params(c, par, lj, part.as<double>()), out);
+    double h[2];
+    NBX_HIP(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
+    *ke = h[0];
+    *pe = h[1];
+    return NBX_OK;
+}
+
+}  // namespace
+
+int mx_learn_energy(nbx_ctx* c, double* ke, double* pe) {
+    mx_plan p;
+    lj_consts lj;
+    int rc;
+    if ((rc = learn_plan(c, p, lj))) return rc;
+    if ((rc = fresh_list(c, p))) return rc;
+    dev_buf part;
+    if ((rc = dalloc(c, part, sizeof(double) * 2 * lr_energy_blocks(c->n, c->nl.lanes)))) return rc;
+    return list_energy(c, 0, lj, part, ke, pe);
+}
+
+int mx_features(nbx_ctx* c, double* feats, double* frames) {
+    mx_plan p;
+    lj_consts lj;
+    int rc;
+    if ((rc = learn_plan(c, p, lj))) return rc;
+    if ((rc = fresh_list(c, p))) return rc;
+    const int64_t n = c->n;
+    dev_buf f, fr;
+    if ((rc = dalloc(c, f, sizeof(float) * LR_NIN * n))) return rc;
+    if ((rc = dalloc(c, fr, sizeof(float) * 6 * n))) return rc;
+    lr_feat_params fp = feat_params(c, 0);
+    fp.fraw = f.as<float>();
+    fp.frames = fr.as<float>();
+    lr_launch_features(c->stream, c->nl.lanes, fp);
+    NBX_HIP(c, hipGetLastError());
+    std::vector<float> hf((size_t)LR_NIN * n), hr((size_t)6 * n);
+    NBX_HIP(c, hipMemcpyAsync(hf.data(), f.p, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, c->stream));
+    NBX_HIP(c, hipMemcpyAsync(hr.data(), fr.p, sizeof(float) * hr.size(), hipMemcpyDeviceToHost, c->stream));
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
+    if (feats)
+        for (size_t k = 0; k < hf.size(); ++k) feats[k] = hf[k];
+    if (frames)
+        for (size_t k = 0; k < hr.size(); ++k) frames[k] = hr[k];
+    return NBX_OK;
+}
+
+int mx_learn_collect(nbx_ctx* c, nbx_model* m, double dt, int stride, int64_t nframes, int nsel,
+                     uint64_t seed) {
+    mx_plan p;
+    lj_consts lj;
+    int rc;
+    if ((rc = learn_plan(c, p, lj))) return rc;
+    if (m->device != c->device)
+        return nbx_fail(c, NBX_ERR_INVALID, "model and context live on different devices");
+    if ((rc = lr_model_reserve(m, m->count + nframes * (int64_t)nsel)))
+        return nbx_fail(c, rc, m->err);
+    dev_buf sel, fst, frm, snap;
+    if ((rc = dalloc(c, sel, sizeof(int) * nsel))) return rc;
+    if ((rc = dalloc(c, fst, sizeof(float) * LR_NIN * nsel))) return rc;
+    if ((rc = dalloc(c, frm, sizeof(float) * 6 * nsel))) return rc;
+    if ((rc = dalloc(c, snap, sizeof(double) * 9 * nsel))) return rc;
+    hipStream_t s = c->stream;
+    const double D = dt * stride;
+    for (int64_t f = 0; f < nframes; ++f) {
+        if ((rc = fresh_list(c, p))) return rc;
+        lr_launch_select(s, sel.as<int>(), nsel, c->n, seed, m->frames_collected + f);
+        lr_feat_params fp = feat_params(c, 0);
+        fp.sel = sel.as<int>();
+        fp.nsel = nsel;
+        fp.fraw = fst.as<float>();
+        fp.frames = frm.as<float>();
+        fp.snap_x = snap.as<double>();
+        lr_launch_features(s, c->nl.lanes, fp);
+        NBX_HIP(c, hipGetLastError());
+        int64_t nf = 0;
+        if ((rc = mx_vv(c, dt, stride, seed, 0, nullptr, nullptr, &nf))) return rc;
+        lr_target_params tp{};
+        tp.sel = sel.as<int>();
+        tp.nsel = nsel;
+        tp.n = (int)c->n;
+        tp.snap = snap.as<double>();
+        tp.x1 = c->d_x;
+        tp.v1 = c->d_v;
+        tp.a1 = c->d_a;
+        tp.D = D;
+        tp.frames = frm.as<float>();
+        tp.fstage = fst.as<float>();
+        tp.X = m->d_X;
+        tp.T = m->d_T;
+        tp.row0 = m->count;
+        lr_launch_targets(s, tp);
+        NBX_HIP(c, hipGetLastError());
+        m->count += nsel;
+    }
+    NBX_HIP(c, hipStreamSynchronize(s));
+    m->frames_collected += nframes;
+    return NBX_OK;
+}
+
+// report: [0] strided steps taken, [1] plain steps taken after a trip, [2] tripped (0/1),
+// [3] strided steps done before the trip, [4] E0, [5] final E, [6] largest relative
+// deviation seen by the monitor, [7] list builds during the run
+int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_t nbig,
+                   double tol, int check_every, double* report) {
+    mx_plan p;
+    lj_consts lj;
+    int rc;
+    if ((rc = learn_plan(c, p, lj))) return rc;
+    if (m->device != c->device)
+        return nbx_fail(c, NBX_ERR_INVALID, "model and context live on different devices");
+    if (!m->norm_set) return nbx_fail(c, NBX_ERR_INVALID, "the model has no normalization");
+    const int64_t n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    const int64_t ntiles = (n + 31) / 32;
+    hipStream_t s = c->stream;
+    dev_buf ffrag, frames, dlab, mpart, epart, bk;
+    if ((rc = dalloc(c, ffrag, 2 * (size_t)ntiles * 3 * LR_FRAG_ELEMS))) return rc;
+    if ((rc = dalloc(c, frames, sizeof(float) * 6 * n))) return rc;
+    if ((rc = dalloc(c, dlab, sizeof(float) * 6 * n))) return rc;
+    if ((rc = dalloc(c, mpart, sizeof(double) * 6 * 256))) return rc;
+    if ((rc = dalloc(c, bk, sizeof(double) * 3 * n3))) return rc;
+    NBX_HIP(c, hipMemsetAsync(ffrag.p, 0, 2 * (size_t)ntiles * 3 * LR_FRAG_ELEMS, s));
+    if ((rc = fresh_list(c, p))) return rc;
+    // energy-partials buffer sized for the largest lane count the lists may use
+    if ((rc = dalloc(c, epart, sizeof(double) * 2 * lr_energy_blocks(n, 16)))) return rc;
+    double ke0, pe0;
+    if ((rc = list_energy(c, 0, lj, epart, &ke0, &pe0))) return rc;
+    const double E0 = ke0 + pe0;
+    int hflags[8];
+    if ((rc = read_flags(c, hflags))) return rc;
+    const int builds0 = hflags[2];
+    const double D = dt * stride;
+    const int nblk = lr_infer_blocks(n);
+
+    // the corrected start of a strided step from the state in X[par] / P[par]
+    auto start_step = [&](int par) {
+        lr_feat_params fp = feat_params(c, par);
+        fp.frames = frames.as<float>();
+        fp.ffrag = ffrag.as<uint16_t>();
+        fp.norm = m->d_norm;
+        lr_launch_features(s, c->nl.lanes, fp);
+        lr_infer_params ip{};
+        ip.ffrag = ffrag.as<uint16_t>();
+        ip.frames = frames.as<float>();
+        ip.params = m->d_params;
+        ip.norm = m->d_norm;
+        ip.frag = m->d_frag;
+        ip.n = (int)n;
+        ip.dlab = dlab.as<float>();
+        ip.part = mpart.as<double>();
+        lr_launch_mlp_infer(s, ip);
+        lr_launch_drift(s, mpart.as<double>(), nblk, dlab.as<float>(), D, make_io(c, p, par, 1), n);
+        after_drift(c, p, par ^ 1, 0);
+    };
+    auto run_chunk = [&](int64_t msteps) -> int {
+        int rc2;
+        if ((rc2 = fresh_list(c, p))) return rc2;
+        mx_launch_fill_int(s, req_flag(c, 0), 1, 0);  // consumed by the fresh build
+        mx_launch_fill_int(s, req_flag(c, 1), 1, 0);
+        start_step(0);
+        const mx_kick_params kp = kick_params(c, 1, D, 0);
+        for (int64_t k = 1; k <= msteps; ++k) {
+            const int par = (int)(k & 1);
+            launch_build(c, p, par);
+            launch_step(c, p, kp, par, 0);
+            start_step(par);
+        }
+        NBX_HIP(c, hipGetLastError());
+        if (msteps & 1)
+            NBX_HIP(c, hipMemcpyAsync(c->d_x, c->d_x2, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        return NBX_OK;
+    };
+
+    int64_t done = 0, plain = 0, trip_at = -1;
+    int tripped = 0;
+    double maxdev = 0.0, E = E0;
+    double grow = 1.0;
+    const int64_t every = std::max(1, check_every);
+    while (done < nbig) {
+        const int64_t msteps = std::min<int64_t>(every, nbig - done);
+        const double* src[3] = {c->d_x, c->d_v, c->d_a};
+        for (int k = 0; k < 3; ++k)
+            NBX_HIP(c, hipMemcpyAsync(bk.as<double>() + k * n3, src[k], sizeof(double) * n3,
+                                      hipMemcpyDeviceToDevice, s));
+        auto restore = [&]() -> int {
+            double* dst[3] = {c->d_x, c->d_v, c->d_a};
+            for (int k = 0; k < 3; ++k)
+                NBX_HIP(c, hipMemcpyAsync(dst[k], bk.as<double>() + k * n3, sizeof(double) * n3,
+                                          hipMemcpyDeviceToDevice, s));
+            return NBX_OK;
+        };
+        for (int attempt = 0;; ++attempt) {
+            if ((rc = run_chunk(msteps))) return rc;
+            if ((rc = read_flags(c, hflags))) return rc;
+            if (!hflags[1]) break;
+            if (attempt > 8) return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow replay did not converge");
+            if ((rc = restore())) return rc;
+            if ((rc = grow_lists(c, p, grow))) return rc;
+            if ((rc = prologue(c, p))) return rc;
+        }
+        double ke, pe;
+        if ((rc = list_energy(c, (int)(msteps & 1), lj, epart, &ke, &pe))) return rc;
+        E = ke + pe;
+        const double dev = std::fabs(E - E0) / std::fabs(E0);
+        const bool bad = !(dev <= tol);  // also catches NaN
+        if (!bad) {
+            maxdev = std::max(maxdev, dev);
+            done += msteps;
+            c->steps_taken += msteps * stride;
+            continue;
+        }
+        // the monitor tripped: back to the chunk's start, plain velocity Verlet from there
+        tripped = 1;
+        trip_at = done;
+        if ((rc = restore())) return rc;
+        c->accel_valid = true;
+        plain = (nbig - done) * stride;
+        int64_t nf = 0;
+        if ((rc = mx_vv(c, dt, plain, 0, 0, nullptr, nullptr, &nf))) return rc;
+        if ((rc = fresh_list(c, p))) return rc;
+        if ((rc = list_energy(c, 0, lj, epart, &ke, &pe))) return rc;
+        E = ke + pe;
+        maxdev = std::max(maxdev, std::fabs(E - E0) / std::fabs(E0));
+        break;
+    }
+    if ((rc = read_flags(c, hflags))) return rc;
+    if (report) {
+        report[0] = (double)done;
+        report[1] = (double)plain;
+        report[2] = (double)tripped;
+        report[3] = (double)trip_at;
+        report[4] = E0;
+        report[5] = E;
+        report[6] = maxdev;
+        report[7] = (double)(hflags[2] - builds0);
+    }
+    return NBX_OK;
+}

@@ -4,7 +4,7 @@ Same names, fields and argument meaning as the Julia package (module export list
 src/NBodySimulator.jl:26-36); the force evaluation, the velocity-Verlet step, the
 thermostats, energies, temperature and rdf run as HIP kernels on the GPU.
 """
-from ._lib import LIB_PATH, Batch, Context, NBXError, device_count, load
+from ._lib import LIB_PATH, Batch, Context, Model, NBXError, device_count, load
 from .simulation import (MI355XVerlet, SimulationResult, Solution, VelocityVerlet,
                          build_context, get_accelerating_function,
                          get_group_accelerating_function, get_position, get_velocity,

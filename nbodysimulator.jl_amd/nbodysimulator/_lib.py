@@ -62,6 +62,7 @@ SIGNATURES = {
     "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
+    "nbx_copy_probe": (C.c_int, [_P, _I64, _I32, _DP]),
     "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
     "nbx_batch_create": (C.c_int, [C.POINTER(_P), _I32, C.POINTER(_I64), _DP, _DP, _D, _D,
                                    _D, _D]),
@@ -74,7 +75,29 @@ SIGNATURES = {
     "nbx_batch_run": (C.c_int, [_P, _D, _I64, _U64, _I64]),
     "nbx_batch_energies": (C.c_int, [_P, _DP, _DP, C.POINTER(_I64)]),
     "nbx_batch_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
+    "nbx_model_create": (C.c_int, [C.POINTER(_P), _U64]),
+    "nbx_model_destroy": (C.c_int, [_P]),
+    "nbx_model_last_error": (C.c_char_p, [_P]),
+    "nbx_model_set": (C.c_int, [_P, _DP, _DP]),
+    "nbx_model_get": (C.c_int, [_P, _DP, _DP]),
+    "nbx_model_forward": (C.c_int, [_P, _DP, _I64, _DP]),
+    "nbx_model_gradient": (C.c_int, [_P, _DP, _DP, _I64, _DP, _DP]),
+    "nbx_model_add_samples": (C.c_int, [_P, _DP, _DP, _I64]),
+    "nbx_model_get_samples": (C.c_int, [_P, _I64, _I64, _DP, _DP, C.POINTER(_I64)]),
+    "nbx_model_clear_samples": (C.c_int, [_P]),
+    "nbx_model_fit_normalization": (C.c_int, [_P]),
+    "nbx_model_train": (C.c_int, [_P, _I64, _I32, _D, _U64, _DP]),
+    "nbx_model_evaluate": (C.c_int, [_P, _I64, _I64, _DP, _DP]),
+    "nbx_features": (C.c_int, [_P, _DP, _DP]),
+    "nbx_learn_energy": (C.c_int, [_P, _DP, _DP]),
+    "nbx_learn_collect": (C.c_int, [_P, _P, _D, _I32, _I64, _I32, _U64]),
+    "nbx_learned_run": (C.c_int, [_P, _P, _D, _I32, _I64, _D, _I32, _DP]),
 }
+
+MLP_INPUTS = 35
+MLP_OUTPUTS = 6
+MLP_PARAMS = 38406
+MLP_NORM = 76
 
 _lib = None
 
@@ -272,6 +295,39 @@ class Context:
         self._check(self._lib.nbx_count_pairs(self._h, C.byref(p)))
         return p.value
 
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """Average duration (ms) of a plain copy kernel moving `nbytes` (read + write)."""
+        ms = C.c_double()
+        self._check(self._lib.nbx_copy_probe(self._h, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
+
+    # -- learned stride -------------------------------------------------------
+    def features(self):
+        """Descriptor of the current state: (feats (n, 35), frames (n, 6))."""
+        f = np.empty((self.n, MLP_INPUTS))
+        fr = np.empty((self.n, 6))
+        self._check(self._lib.nbx_features(self._h, dptr(f), dptr(fr)))
+        return f, fr
+
+    def learn_energy(self):
+        """(kinetic, shifted Lennard-Jones potential) energy from the Verlet list."""
+        ke, pe = C.c_double(), C.c_double()
+        self._check(self._lib.nbx_learn_energy(self._h, C.byref(ke), C.byref(pe)))
+        return ke.value, pe.value
+
+    def learn_collect(self, model, dt, stride, nframes, atoms_per_frame, seed=0):
+        self._check(self._lib.nbx_learn_collect(self._h, model._h, float(dt), int(stride),
+                                                int(nframes), int(atoms_per_frame), int(seed)))
+
+    def learned_run(self, model, dt, stride, nsteps, drift_tol=1e-4, check_every=25) -> dict:
+        r = np.zeros(8)
+        self._check(self._lib.nbx_learned_run(self._h, model._h, float(dt), int(stride),
+                                              int(nsteps), float(drift_tol), int(check_every),
+                                              dptr(r)))
+        return dict(strided_steps=int(r[0]), plain_steps=int(r[1]), tripped=bool(r[2]),
+                    strided_before_trip=int(r[3]), E0=r[4], E=r[5], max_rel_dev=r[6],
+                    list_builds=int(r[7]))
+
     def stats(self) -> dict:
         out = (C.c_int64 * 5)()
         self._check(self._lib.nbx_get_stats(self._h, out, 5))
@@ -351,3 +407,101 @@ class Batch:
 
 def device_count() -> int:
     return int(load().nbx_device_count())
+
+
+class Model:
+    """Owns one nbx_model: the learned-stride MLP, its normalization and training set."""
+
+    def __init__(self, seed: int = 0):
+        lib = load()
+        self._lib = lib
+        h = _P()
+        rc = lib.nbx_model_create(C.byref(h), int(seed))
+        if rc != NBX_OK:
+            raise_for(rc, lib.nbx_model_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != NBX_OK:
+            raise_for(rc, self._lib.nbx_model_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbx_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self):
+        p = np.empty(MLP_PARAMS)
+        nm = np.empty(MLP_NORM)
+        self._check(self._lib.nbx_model_get(self._h, dptr(p), dptr(nm)))
+        return p, nm
+
+    def set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        nm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float64)
+        if p is not None and p.shape != (MLP_PARAMS,):
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        if nm is not None and nm.shape != (MLP_NORM,):
+            raise ValueError(f"expected {MLP_NORM} normalization values")
+        self._check(self._lib.nbx_model_set(self._h, dptr(p), dptr(nm)))
+
+    def forward(self, feats):
+        f = np.ascontiguousarray(feats, dtype=np.float64).reshape(-1, MLP_INPUTS)
+        out = np.empty((f.shape[0], MLP_OUTPUTS))
+        self._check(self._lib.nbx_model_forward(self._h, dptr(f), f.shape[0], dptr(out)))
+        return out
+
+    def gradient(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float64).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, MLP_OUTPUTS)
+        g = np.empty(MLP_PARAMS)
+        loss = C.c_double()
+        self._check(self._lib.nbx_model_gradient(self._h, dptr(f), dptr(t), f.shape[0], dptr(g),
+                                                 C.byref(loss)))
+        return g, loss.value
+
+    def add_samples(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float64).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, MLP_OUTPUTS)
+        if f.shape[0] != t.shape[0]:
+            raise ValueError("feature and target row counts differ")
+        self._check(self._lib.nbx_model_add_samples(self._h, dptr(f), dptr(t), f.shape[0]))
+
+    @property
+    def num_samples(self) -> int:
+        c = C.c_int64()
+        self._check(self._lib.nbx_model_get_samples(self._h, 0, 0, None, None, C.byref(c)))
+        return c.value
+
+    def get_samples(self, first=0, n=None):
+        if n is None:
+            n = self.num_samples - first
+        f = np.empty((n, MLP_INPUTS))
+        t = np.empty((n, MLP_OUTPUTS))
+        self._check(self._lib.nbx_model_get_samples(self._h, int(first), int(n), dptr(f), dptr(t),
+                                                    None))
+        return f, t
+
+    def clear_samples(self):
+        self._check(self._lib.nbx_model_clear_samples(self._h))
+
+    def fit_normalization(self):
+        self._check(self._lib.nbx_model_fit_normalization(self._h))
+
+    def train(self, iters, batch=4096, lr=1e-3, seed=0):
+        losses = np.empty(max(int(iters), 1))
+        self._check(self._lib.nbx_model_train(self._h, int(iters), int(batch), float(lr), int(seed),
+                                              dptr(losses)))
+        return losses[:int(iters)]
+
+    def evaluate(self, first, n):
+        mse = np.empty(MLP_OUTPUTS)
+        base = np.empty(MLP_OUTPUTS)
+        self._check(self._lib.nbx_model_evaluate(self._h, int(first), int(n), dptr(mse), dptr(base)))
+        return mse, base

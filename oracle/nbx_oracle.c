@@ -689,3 +689,59 @@ void orc_rdf(const double* frames, int64_t nframes, int64_t n, const int64_t* in
     }
     free(hist);
 }
+
+/* ---------------------------------------------------------------------------
+ * Learned-stride descriptor (definition in nbx_oracle.h; no reference counterpart)
+ * ------------------------------------------------------------------------- */
+static void lr_frame(const double* a, const double* v, double* e1, double* e2, double* ff) {
+    const double an = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (an > 0.0) { e1[0] = a[0] / an; e1[1] = a[1] / an; e1[2] = a[2] / an; }
+    else { e1[0] = 1.0; e1[1] = 0.0; e1[2] = 0.0; }
+    const double vp = v[0] * e1[0] + v[1] * e1[1] + v[2] * e1[2];
+    const double w[3] = {v[0] - vp * e1[0], v[1] - vp * e1[1], v[2] - vp * e1[2]};
+    const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (wn > 0.0) { e2[0] = w[0] / wn; e2[1] = w[1] / wn; e2[2] = w[2] / wn; }
+    else {
+        double c[3] = {0.0, 0.0, 0.0};
+        const double ux = fabs(e1[0]), uy = fabs(e1[1]), uz = fabs(e1[2]);
+        if (ux <= uy && ux <= uz) c[0] = 1.0; else if (uy <= uz) c[1] = 1.0; else c[2] = 1.0;
+        const double cp = c[0] * e1[0] + c[1] * e1[1] + c[2] * e1[2];
+        const double q[3] = {c[0] - cp * e1[0], c[1] - cp * e1[1], c[2] - cp * e1[2]};
+        const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        e2[0] = q[0] / qn; e2[1] = q[1] / qn; e2[2] = q[2] / qn;
+    }
+    ff[0] = vp;
+    ff[1] = wn;
+    ff[2] = an;
+}
+
+void orc_lr_features(const double* u, const double* a, const double* v, int64_t n, double L,
+                     double* feats, double* frames) {
+    orc_bc bc;
+    memset(&bc, 0, sizeof(bc));
+    bc.kind = ORC_BC_CUBIC;
+    bc.b[0] = L;
+    const double pi = 3.14159265358979323846;
+    for (int64_t i = 0; i < n; ++i) {
+        double g[32];
+        for (int b = 0; b < 32; ++b) g[b] = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double rij[3], r, r2;
+            orc_interparticle_distance(u + 3 * i, u + 3 * j, &bc, rij, &r, &r2);
+            if (!(r2 < 6.25)) continue;
+            const double sw = r < 2.0 ? 1.0 : 0.5 * (1.0 + cos(pi * (r - 2.0) / 0.5));
+            for (int b = 0; b < 32; ++b) {
+                const double e = r - (0.8 + 1.7 * (double)b / 31.0);
+                g[b] += sw * exp(-0.5 * e * e / 0.01);
+            }
+        }
+        double e1[3], e2[3], ff[3];
+        lr_frame(a + 3 * i, v + 3 * i, e1, e2, ff);
+        for (int b = 0; b < 32; ++b) feats[35 * i + b] = g[b];
+        feats[35 * i + 32] = ff[0];
+        feats[35 * i + 33] = ff[1];
+        feats[35 * i + 34] = ff[2];
+        for (int k = 0; k < 3; ++k) { frames[6 * i + k] = e1[k]; frames[6 * i + 3 + k] = e2[k]; }
+    }
+}

@@ -119,6 +119,16 @@ double orc_potential_energy(const orc_system* s, const double* u);
 void orc_rdf(const double* frames, int64_t nframes, int64_t n, const int64_t* indxs,
              int64_t nidx, double L, int64_t maxbin, double* rs, double* gr);
 
+/* Learned-stride descriptor (no reference counterpart): for every atom of a cubic box of
+ * side L, 32 Gaussian radial bins (centres 0.8 + 1.7 b / 31, standard deviation 0.1,
+ * cosine switch 1 -> 0 over r in [2.0, 2.5), nothing beyond 2.5) over all other atoms at
+ * the minimum-image distance, then v.e1, |v - (v.e1) e1|, |a| in the force frame
+ * e1 = a/|a| (x axis if a = 0), e2 = unit part of v normal to e1 (if v is parallel to
+ * e1: the axis least aligned with e1, made normal to it).  u, a, v: n x 3 rows;
+ * feats: n x 35; frames: n x 6 (e1, e2). */
+void orc_lr_features(const double* u, const double* a, const double* v, int64_t n, double L,
+                     double* feats, double* frames);
+
 /* Philox4x32-10 (Salmon et al., SC'11) and the derived samplers the GPU uses */
 void orc_philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                     uint32_t k1, uint32_t* out4);

@@ -72,6 +72,7 @@ def load():
         "orc_rdf": (None, [_DP, C.c_int64, C.c_int64, _LP, C.c_int64, C.c_double, C.c_int64,
                            _DP, _DP]),
         "orc_philox4x32": (None, [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]),
+        "orc_lr_features": (None, [_DP, _DP, _DP, C.c_int64, C.c_double, _DP, _DP]),
         "orc_u01": (C.c_double, [C.c_uint32, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
@@ -192,3 +193,14 @@ def philox(c, k):
     load().orc_philox4x32(*[int(x) & 0xFFFFFFFF for x in c], *[int(x) & 0xFFFFFFFF for x in k],
                           out)
     return list(out)
+
+
+def lr_features(u, a, v, L):
+    """Learned-stride descriptor of every atom: (feats (n, 35), frames (n, 6))."""
+    lib = load()
+    u, a, v = c3n(u), c3n(a), c3n(v)
+    n = u.shape[0]
+    f = np.empty((n, 35))
+    fr = np.empty((n, 6))
+    lib.orc_lr_features(dp(u), dp(a), dp(v), n, float(L), dp(f), dp(fr))
+    return f, fr
