@@ -130,7 +130,8 @@ def bench_gravity(args, lib, gen):
                    "n": n, "dt": dt, "steps_in_config": 100,
                    "force_error_vs_fp64_oracle": force_err,
                    "energy_drift_rel": abs(e1 - e0) / abs(e0),
-                   "energy_drift_steps": args.warmup + args.steps},
+                   "energy_drift_steps": args.warmup + args.steps,
+                   "energy_drift_rel_10k": drift10k},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic_from_profiles(kname),
@@ -194,6 +195,186 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
     return out
 
 
+def oracle_system_water(w, Rc):
+    import oracle_lib as O
+    from nbodysimulator import generators as gen
+    P = gen.SPCFW
+    orc = O.System(w.masses, charges=w.charges, water=True, bc=O.make_bc(O.BC_CUBIC, w.L),
+                   kb=P["kb"])
+    orc.water_params((P["eps"], P["sigma"], Rc), O.TERM_RF, (P["k_el"], Rc, math.inf),
+                     P["rOH"], P["aHOH"], P["kbond"], P["kangle"])
+    return orc
+
+
+def water_context(lib, gen, w, Rc, thermostat):
+    P = gen.SPCFW
+    n = len(w.masses)
+    ctx = lib.Context(n, lib.SYS_WATER, w.masses, w.charges)
+    ctx.set_boundary(lib.BC_CUBIC, [w.L, 0, 0, 0, 0, 0])
+    ctx.set_kb(P["kb"])
+    ctx.add_lennard_jones(P["eps"], P["sigma"], Rc)
+    ctx.add_reaction_field(P["k_el"], Rc, math.inf)
+    ctx.set_spcfw(P["rOH"], P["aHOH"], P["kbond"], P["kangle"])
+    ctx.set_precision(lib.PREC_MIXED)
+    if thermostat:
+        ctx.set_thermostat(lib.THERM_BERENDSEN, 300.0, 0.1)
+    ctx.set_state(w.u, w.v)
+    return ctx
+
+
+def bench_water(args, lib, gen):
+    """SPC/Fw water, 4096 molecules (16^3 lattice, random orientations, 0.997 g/cm^3),
+    reaction field and LJ cutoff 0.9 nm, Berendsen 300 K (tau 0.1 ps), dt 0.5 fs."""
+    w = gen.water_box(per_side=16, seed=7)
+    n = len(w.masses)
+    Rc = 0.9
+    dt, steps = 0.0005, 2000
+    ctx = water_context(lib, gen, w, Rc, True)
+    ctx.vv_run(dt, 20)
+    t0 = time.perf_counter()
+    ctx.vv_run(dt, steps)
+    el = time.perf_counter() - t0
+    T1 = ctx.temperature()
+    pairs = ctx.count_pairs()
+    kms, kname = ctx.time_force_kernel(50)
+    # NVE drift of the same system over 10k steps (no thermostat)
+    nve = water_context(lib, gen, w, Rc, False)
+    k0, p0, _ = nve.energies()
+    nve.vv_run(dt, 10000)
+    k1, p1, _ = nve.energies()
+    # force error on the initial state and the CPU baseline (one full RHS of the oracle)
+    orc = oracle_system_water(w, Rc)
+    t0 = time.perf_counter()
+    ref = orc.rhs(w.u)
+    cpu_s = time.perf_counter() - t0
+    fe = water_context(lib, gen, w, Rc, False)
+    a = mixed_accel(fe, w.u)
+    return {"n_atoms": n, "molecules": n // 3, "steps": steps, "dt_ps": dt,
+            "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
+            "pairs_per_step": pairs, "pair_interactions_per_s": pairs * steps / el,
+            "temperature_after_K": T1,
+            "force_kernel": {"name": kname, "avg_ms": kms},
+            "nve_energy_drift_rel_10k_steps": abs((k1 + p1) - (k0 + p0)) / abs(k0 + p0),
+            "force_error_vs_fp64_oracle": float(np.abs(a - ref).max() / np.abs(ref).max()),
+            "cpu_baseline": {"steps_per_s": 1.0 / cpu_s, "cores": 1, "kind": "port",
+                             "sample": f"one FP64 oracle RHS of all {n} atoms (all pairs with "
+                                       f"exclusions, as the reference), {cpu_s:.2f} s"},
+            "gpu_vs_cpu_steps": (steps / el) * cpu_s}
+
+
+def bench_batch(args, lib, gen):
+    """512 LJ replicas of 108..4000 atoms (N = 4 n^3, n ~ U{3..10}), T* ~ U[0.7, 1.5],
+    Langevin (gamma 1) or Andersen (nu 0.01), dt 0.005, 5000 steps in persistent launches."""
+    import oracle_lib as O
+    reps = gen.lj_batch(512, seed=0)
+    systems = [gen.lj_liquid(r.cells, rho=0.8442, T=r.T, seed=k) for k, r in enumerate(reps)]
+    natoms = [s.u.shape[0] for s in systems]
+    b = lib.Batch(natoms, [s.L for s in systems], np.concatenate([s.masses for s in systems]))
+    for k, r in enumerate(reps):
+        if r.thermostat == "langevin":
+            b.set_thermostat(k, lib.THERM_LANGEVIN, r.T, 1.0)
+        else:
+            b.set_thermostat(k, lib.THERM_ANDERSEN, r.T, 0.01)
+    b.set_state(np.concatenate([s.u for s in systems]), np.concatenate([s.v for s in systems]))
+    steps = 5000
+    b.run(0.005, 10)
+    t0 = time.perf_counter()
+    b.run(0.005, steps)
+    el = time.perf_counter() - t0
+    ke, pe, pairs = b.energies()
+    T = 2.0 * ke / (3.0 * np.asarray(natoms))
+    Tt = np.array([r.T for r in reps])
+    # CPU baseline: the oracle's all-pairs RHS of 8 replicas, scaled by sum N^2
+    pick = np.random.default_rng(1).choice(512, 8, replace=False)
+    t0 = time.perf_counter()
+    for k in pick:
+        s = systems[k]
+        O.System(s.masses, bc=O.make_bc(O.BC_CUBIC, s.L)).add(O.TERM_LJ, 1.0, 1.0, 2.5).rhs(s.u)
+    cpu_s = time.perf_counter() - t0
+    n2 = np.asarray(natoms, dtype=np.float64) ** 2
+    cpu_step_s = cpu_s * n2.sum() / n2[pick].sum()
+    return {"replicas": 512, "atoms": int(sum(natoms)), "steps": steps,
+            "ms_per_step": 1e3 * el / steps, "atom_steps_per_s": sum(natoms) * steps / el,
+            "pairs_per_step": int(pairs.sum()),
+            "pair_interactions_per_s": float(pairs.sum()) * steps / el,
+            "temperature_rel_error_median": float(np.median(np.abs(T / Tt - 1.0))),
+            "cpu_baseline": {"steps_per_s": 1.0 / cpu_step_s, "cores": 1, "kind": "port",
+                             "sample": f"FP64 oracle all-pairs RHS of 8 replicas ({cpu_s:.2f} s), "
+                                       f"scaled by sum N^2 to all 512"},
+            "gpu_vs_cpu_steps": (steps / el) * cpu_step_s}
+
+
+def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000):
+    """Learned 8-dt stride on LJ N=32,000 (FCC 20^3, rho* 0.8442, T* 0.9, NVE): training
+    set from `ntraj` plain trajectories of `traj_steps` steps (256 atoms per 8-step frame),
+    Adam training, then 10k dt-steps of simulated time with the learned stride (energy
+    monitor every 25 strided steps, fallback at 1e-4) against plain velocity Verlet."""
+    dt, stride = 0.005, 8
+
+    def ctx_for(seed):
+        s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=seed)
+        ctx = lib.Context(s.u.shape[0], lib.SYS_POTENTIAL, s.masses)
+        ctx.set_boundary(lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+        ctx.add_lennard_jones(1.0, 1.0, 2.5)
+        ctx.set_kb(1.0)
+        ctx.set_precision(lib.PREC_MIXED)
+        ctx.set_state(s.u, s.v)
+        return ctx, s
+
+    model = lib.Model(1)
+    heldm = lib.Model(1)
+    nheld = max(2, ntraj // 20)
+    t0 = time.perf_counter()
+    gctx, _ = ctx_for(0)
+    for tr in range(ntraj + nheld):
+        s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=1000 + tr)
+        gctx.set_state(s.u, s.v)
+        gctx.learn_collect(model if tr < ntraj else heldm, dt, stride, traj_steps // stride, 256,
+                           seed=tr)
+    t_collect = time.perf_counter() - t0
+    nsamp = model.num_samples
+    model.fit_normalization()
+    t0 = time.perf_counter()
+    losses = model.train(iters, batch=8192, lr=1e-3, seed=7)
+    t_train = time.perf_counter() - t0
+    # the plain integrator judges the strided step on trajectories never trained on
+    heldm.set(*model.get())
+    mse, base = heldm.evaluate(0, heldm.num_samples)
+    # evaluation: 10k dt-steps of simulated time from a fresh seeded state
+    plain, s = ctx_for(4242)
+    ke0, pe0 = plain.learn_energy()
+    plain.vv_run(dt, 20)
+    plain.set_state(s.u, s.v)
+    t0 = time.perf_counter()
+    plain.vv_run(dt, 10000)
+    t_plain = time.perf_counter() - t0
+    ke1, pe1 = plain.learn_energy()
+    up, _ = plain.get_state()
+    learned, _ = ctx_for(4242)
+    t0 = time.perf_counter()
+    rep = learned.learned_run(model, dt, stride, 10000 // stride, drift_tol=1e-4, check_every=25)
+    t_learned = time.perf_counter() - t0
+    ul, _ = learned.get_state()
+    rs, g_plain = plain.rdf(up[None], 200) if hasattr(plain, "rdf") else (None, None)
+    _, g_learned = learned.rdf(ul[None], 200) if hasattr(learned, "rdf") else (None, None)
+    sim_time = 10000 * dt
+    e0 = ke0 + pe0
+    return {"n": 32000, "dt": dt, "stride": stride,
+            "training": {"trajectories": ntraj, "steps_each": traj_steps, "samples": nsamp,
+                         "heldout_trajectories": nheld,
+                         "collect_s": t_collect, "iters": iters, "batch": 8192,
+                         "train_s": t_train, "final_loss": float(np.mean(losses[-50:])),
+                         "heldout_mse_over_uncorrected": (mse / base).tolist()},
+            "plain": {"sim_time_per_s": sim_time / t_plain, "steps_per_s": 10000 / t_plain,
+                      "energy_drift_rel_10k": abs((ke1 + pe1) - e0) / abs(e0)},
+            "learned": {"sim_time_per_s": sim_time / t_learned, "report": rep,
+                        "energy_drift_rel_10k": abs(rep["E"] - rep["E0"]) / abs(rep["E0"]),
+                        "fraction_of_time_strided": rep["strided_steps"] * stride / 10000.0},
+            "speedup_sim_time": t_plain / t_learned,
+            "rdf_max_abs_diff": (float(np.abs(np.asarray(g_plain) - np.asarray(g_learned)).max())
+                                 if g_plain is not None else None)}
+
+
 def profile_kernels(lib, gen):
     """Short launch sequence for hardware-counter runs: the gravity force kernel at the
     headline size and a plain copy kernel of known size (the byte-counter calibration)."""
@@ -218,6 +399,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--learn-traj", type=int, default=200,
+                    help="training trajectories of the learned-stride line")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the dominant kernels (gravity N=65536 force, 10 launches; "
                          "a 1 GiB copy-kernel calibration, 5 launches) for rocprofv3 counters")
@@ -255,10 +438,17 @@ def main():
     }
     if not args.no_secondary:
         log("[bench] LJ N=32000")
-        res["secondary"] = {"lj_n32000_T0.9": bench_lj(args, lib, gen, 20, 0.9, 2000, 600)}
+        res["secondary"] = {"lj_n32000_T0.9": bench_lj(args, lib, gen, 20, 0.9, 10000, 600)}
         log("[bench] LJ N=864")
-        res["secondary"]["lj_n864_T0.72"] = bench_lj(args, lib, gen, 6, 0.72, 1000, 864,
+        res["secondary"]["lj_n864_T0.72"] = bench_lj(args, lib, gen, 6, 0.72, 10000, 864,
                                                      fp64=True)
+        log("[bench] water 4096 molecules")
+        res["secondary"]["water_spcfw_4096"] = bench_water(args, lib, gen)
+        log("[bench] batch 512 replicas")
+        res["secondary"]["lj_batch_512"] = bench_batch(args, lib, gen)
+        log("[bench] learned stride N=32000")
+        res["secondary"]["lj_n32000_learned_stride"] = bench_learned(args, lib, gen,
+                                                                     ntraj=args.learn_traj)
     print(json.dumps(res), flush=True)
     return 0
 

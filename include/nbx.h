@@ -169,6 +169,13 @@ int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
  * launches on the context's stream.  Calibrates HBM byte counters (the kernel reads
  * and writes exactly `bytes` each) and gives the achievable bandwidth. */
 int nbx_copy_probe(nbx_ctx* ctx, int64_t bytes, int32_t iters, double* avg_ms);
+/* Autotuner (no reference counterpart): times `trial_steps` mixed-path steps of length dt
+ * from the current state for every candidate launch shape and keeps the fastest -- list
+ * plans: lanes per atom {2, 4, 8, 16} x Verlet skin {0.06 .. 0.24} x cutoff; gravity:
+ * targets per thread {1, 2} x source chunks {1 .. 64}.  The state and step count are
+ * restored.  out (nout <= 5): [0] lanes | targets per thread, [1] skin | chunks, [2] best
+ * ms per step, [3] ms per step of the configuration on entry, [4] candidates timed. */
+int nbx_autotune(nbx_ctx* ctx, double dt, int32_t trial_steps, double* out, int32_t nout);
 /* Diagnostics of the mixed path (no reference counterpart): out[0] Verlet-list builds
  * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),
  * [3] lanes per atom in the force kernel, [4] steps per captured graph.  nout <= 5. */

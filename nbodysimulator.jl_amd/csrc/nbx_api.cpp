@@ -726,6 +726,17 @@ extern "C" int nbx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int32_
     return mx_learned_run(c, m, dt, stride, nsteps, drift_tol, check_every, report);
 }
 
+extern "C" int nbx_autotune(nbx_ctx* c, double dt, int32_t trial_steps, double* out,
+                            int32_t nout) {
+    if (!c || (nout > 0 && !out) || nout < 0 || nout > 5) return NBX_ERR_INVALID;
+    if (!c->state_set) return nbx_fail(c, NBX_ERR_INVALID, "no state set");
+    if (c->prec != NBX_PREC_MIXED)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "autotuning is provided for the mixed path");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return nbx_fail(c, NBX_ERR_INVALID, "dt must be > 0");
+    if (trial_steps < 1) return nbx_fail(c, NBX_ERR_INVALID, "trial_steps must be >= 1");
+    return mx_autotune(c, dt, trial_steps, out, nout);
+}
+
 extern "C" int nbx_get_stats(nbx_ctx* c, int64_t* out, int32_t nout) {
     if (!c || !out || nout < 0 || nout > 5) return NBX_ERR_INVALID;
     int64_t v[5] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps};

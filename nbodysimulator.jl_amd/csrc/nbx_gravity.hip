@@ -107,12 +107,13 @@ k_grav_finish(const double* __restrict__ part, int chunks, int n, double G, mx_k
 }
 
 static int grav_tpt(int64_t n) { return n >= 16384 ? 2 : 1; }
+static int grav_tpt_of(const nbx_ctx* c) { return c->grav_tpt_set ? c->grav_tpt_set : grav_tpt(c->n); }
 
 void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double G,
                        const mx_kick_params& kp, const mx_io& io, int force_only_kernel) {
     const int n = (int)c->n;
     if (n <= 0) return;
-    const int tpt = grav_tpt(n);
+    const int tpt = grav_tpt_of(c);
     const int chunks = c->grav_chunks;
     const int jchunk = (int)((n + chunks - 1) / chunks);
     const dim3 grid(nbx_blocks(n, GR_THREADS * tpt), chunks);
@@ -129,8 +130,10 @@ void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double 
 
 // number of source chunks: enough blocks to fill 256 CUs several times over, with
 // chunks of at least 2048 sources
-int mx_gravity_chunks(int64_t n) {
-    const int tpt = grav_tpt(n);
+int mx_gravity_chunks(const nbx_ctx* c) {
+    const int64_t n = c->n;
+    if (c->grav_chunks_set) return c->grav_chunks_set;
+    const int tpt = grav_tpt_of(c);
     const int64_t tblocks = (n + GR_THREADS * tpt - 1) / (GR_THREADS * tpt);
     int chunks = 1;
     while (tblocks * chunks < 2048 && (n / (chunks * 2)) >= 2048) chunks *= 2;

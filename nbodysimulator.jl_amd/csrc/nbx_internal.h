@@ -88,6 +88,8 @@ struct nbx_ctx {
     double* d_x2 = nullptr;   // second unwrapped-position buffer (odd steps)
     double* d_part = nullptr; // gravity: per-chunk partial accelerations
     int grav_chunks = 1;
+    int grav_tpt_set = 0;     // targets per thread of the gravity kernel (0: by size)
+    int grav_chunks_set = 0;  // source chunks of the gravity kernel (0: by size)
     nbx_nlist nl;
     int64_t* d_step = nullptr;       // absolute step base read by replayed graphs
     hipGraphExec_t gexec = nullptr;  // captured block of mixed-path steps
@@ -170,6 +172,7 @@ int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d
 // ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name);
 int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms);
+int mx_autotune(nbx_ctx* c, double dt, int trial_steps, double* out, int nout);
 // learned stride (nbx_mixed.cpp)
 int mx_features(nbx_ctx* c, double* feats, double* frames);
 int mx_learn_energy(nbx_ctx* c, double* ke, double* pe);

@@ -17,7 +17,7 @@
 
 #include "nbx_mixed.h"
 
-int mx_gravity_chunks(int64_t n);
+int mx_gravity_chunks(const nbx_ctx* c);
 
 namespace {
 
@@ -295,7 +295,7 @@ static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
         NBX_HIP(c, hipMalloc((void**)&c->d_x2, sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1)));
     if (!c->d_step) NBX_HIP(c, hipMalloc((void**)&c->d_step, sizeof(int64_t)));
     if (p.kind == PLAN_GRAVITY) {
-        const int chunks = mx_gravity_chunks(n);
+        const int chunks = mx_gravity_chunks(c);
         if (!c->d_part || c->grav_chunks != chunks) {
             if (c->d_part) (void)hipFree(c->d_part);
             c->d_part = nullptr;
@@ -918,5 +918,133 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         report[6] = maxdev;
         report[7] = (double)(hflags[2] - builds0);
     }
+    return NBX_OK;
+}
+
+// =================================================================================
+// Autotuner (no counterpart in the reference): the mixed path's launch shape chosen by
+// timing.  Each candidate runs `trial` velocity-Verlet steps from the current state after
+// a warm-up run long enough to capture its step graph; the state, the step count and the
+// integrator's inputs are restored after every trial, and the fastest candidate is kept.
+//   list plans   : lanes per atom {2, 4, 8, 16} x skin {0.06, 0.09, 0.12, 0.18, 0.24} x cutoff
+//   gravity      : targets per thread {1, 2} x source chunks {1, 2, 4, ..., 64}
+// out: [0] lanes | targets per thread, [1] skin | chunks, [2] best ms per step,
+//      [3] ms per step of the configuration found on entry, [4] candidates timed.
+// =================================================================================
+#include <chrono>
+
+int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
+    mx_plan p;
+    std::string why;
+    if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+    const bool grav = p.kind == PLAN_GRAVITY;
+    if (!grav && !p.nb.mode_list)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "autotuning covers list and gravity plans");
+    const int64_t n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    hipStream_t s = c->stream;
+    dev_buf bx, bv;
+    int rc;
+    if ((rc = dalloc(c, bx, sizeof(double) * n3))) return rc;
+    if ((rc = dalloc(c, bv, sizeof(double) * n3))) return rc;
+    NBX_HIP(c, hipMemcpyAsync(bx.p, c->d_x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+    NBX_HIP(c, hipMemcpyAsync(bv.p, c->d_v, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+    NBX_HIP(c, hipStreamSynchronize(s));
+    const int64_t steps0 = c->steps_taken;
+    auto restore = [&]() -> int {
+        NBX_HIP(c, hipMemcpyAsync(c->d_x, bx.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        NBX_HIP(c, hipMemcpyAsync(c->d_v, bv.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        NBX_HIP(c, hipStreamSynchronize(s));
+        c->steps_taken = steps0;
+        c->accel_valid = false;
+        return NBX_OK;
+    };
+    // ms per step of the current configuration
+    auto time_config = [&](double* ms) -> int {
+        int r;
+        c->nl.ready = false;
+        c->accel_valid = false;
+        ++c->version;
+        int64_t nf = 0;
+        const int warm = std::max(c->graph_steps + 2, 8);
+        if ((r = restore())) return r;
+        if ((r = mx_vv(c, dt, warm, 0, 0, nullptr, nullptr, &nf))) return r;
+        if ((r = restore())) return r;
+        if ((r = mx_vv(c, dt, 2, 0, 0, nullptr, nullptr, &nf))) return r;  // prologue
+        if ((r = restore())) return r;
+        if ((r = ensure_ready(c, p))) return r;
+        NBX_HIP(c, hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((r = mx_vv(c, dt, trial, 0, 0, nullptr, nullptr, &nf))) return r;
+        const auto t1 = std::chrono::steady_clock::now();
+        *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial;
+        return NBX_OK;
+    };
+    double base_ms = 0.0, best_ms = 0.0;
+    double best0 = 0.0, best1 = 0.0;
+    int count = 0;
+    if (grav) {
+        const int tpt0 = c->grav_tpt_set, ch0 = c->grav_chunks_set;
+        if ((rc = time_config(&base_ms))) return rc;
+        best_ms = base_ms;
+        best0 = tpt0;
+        best1 = ch0;
+        for (int tpt = 1; tpt <= 2; ++tpt)
+            for (int ch = 1; ch <= 64; ch *= 2) {
+                if (n / ch < 256) break;
+                c->grav_tpt_set = tpt;
+                c->grav_chunks_set = ch;
+                double ms;
+                if ((rc = time_config(&ms))) return rc;
+                ++count;
+                if (ms < best_ms) { best_ms = ms; best0 = tpt; best1 = ch; }
+            }
+        c->grav_tpt_set = (int)best0;
+        c->grav_chunks_set = (int)best1;
+        if (!c->grav_tpt_set) best0 = n >= 16384 ? 2 : 1;
+        if (!c->grav_chunks_set) best1 = mx_gravity_chunks(c);
+    } else {
+        const int lanes0 = c->nl.lanes;
+        const double skin0 = c->skin;
+        const bool skin_set0 = c->skin_set;
+        if ((rc = time_config(&base_ms))) return rc;
+        best_ms = base_ms;
+        best0 = lanes0;
+        best1 = c->nl.skin;
+        const int lanes_c[4] = {2, 4, 8, 16};
+        const double skin_c[5] = {0.06, 0.09, 0.12, 0.18, 0.24};
+        bool set_best = false;
+        for (int li = 0; li < 4; ++li)
+            for (int si = 0; si < 5; ++si) {
+                c->nl.lanes = lanes_c[li];
+                c->skin = skin_c[si] * p.cutoff;
+                c->skin_set = true;
+                double ms;
+                rc = time_config(&ms);
+                if (rc == NBX_ERR_CAPACITY) continue;  // too dense a cell for this skin
+                if (rc) return rc;
+                ++count;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best0 = lanes_c[li];
+                    best1 = c->skin;
+                    set_best = true;
+                }
+            }
+        if (set_best) {
+            c->nl.lanes = (int)best0;
+            c->skin = best1;
+            c->skin_set = true;
+        } else {
+            c->nl.lanes = lanes0;
+            c->skin = skin0;
+            c->skin_set = skin_set0;
+        }
+    }
+    if ((rc = restore())) return rc;
+    c->nl.ready = false;
+    ++c->version;
+    const double vals[5] = {best0, best1, best_ms, base_ms, (double)count};
+    for (int k = 0; k < nout && k < 5; ++k) out[k] = vals[k];
     return NBX_OK;
 }

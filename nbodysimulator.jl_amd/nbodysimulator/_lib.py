@@ -64,6 +64,7 @@ SIGNATURES = {
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
     "nbx_copy_probe": (C.c_int, [_P, _I64, _I32, _DP]),
     "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
+    "nbx_autotune": (C.c_int, [_P, _D, _I32, _DP, _I32]),
     "nbx_batch_create": (C.c_int, [C.POINTER(_P), _I32, C.POINTER(_I64), _DP, _DP, _D, _D,
                                    _D, _D]),
     "nbx_batch_destroy": (C.c_int, [_P]),
@@ -300,6 +301,13 @@ class Context:
         ms = C.c_double()
         self._check(self._lib.nbx_copy_probe(self._h, int(nbytes), int(iters), C.byref(ms)))
         return ms.value
+
+    def autotune(self, dt: float, trial_steps: int = 200) -> dict:
+        """Time the candidate launch shapes of the mixed path and keep the fastest."""
+        out = np.zeros(5)
+        self._check(self._lib.nbx_autotune(self._h, float(dt), int(trial_steps), dptr(out), 5))
+        return dict(shape0=out[0], shape1=out[1], best_ms_per_step=out[2],
+                    entry_ms_per_step=out[3], candidates=int(out[4]))
 
     # -- learned stride -------------------------------------------------------
     def features(self):
