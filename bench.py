@@ -14,10 +14,15 @@ The line also carries:
   cpu_baseline -- the FP64 oracle (a restatement of the reference's per-atom loop) on
                   one host core, timed on a bounded sample of targets
   secondary    -- Lennard-Jones N = 32,000 (FCC, rho* = 0.8442, T* = 0.9, r_c = 2.5) and
-                  N = 864 (T* = 0.72) steps/s in the mixed and FP64 paths, with their
-                  CPU-oracle baselines, force errors and energy drift
+                  N = 864 (T* = 0.72) steps/s in the mixed and FP64 paths; SPC/Fw water
+                  (4,096 molecules, reaction field 0.9 nm, Berendsen 300 K); 512 batched
+                  LJ replicas; the learned 8-dt stride on LJ N = 32,000 (training, held-out
+                  error, simulated time per second against plain velocity Verlet at equal
+                  drift, the stride sweep 4 and 2, and each stride without the network);
+                  each with its CPU-oracle baseline, force error and 10k-step energy drift
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary] [--learn-traj T]
+    python bench.py --profile-kernels     (short launch sequence for rocprofv3 counters)
 """
 import argparse
 import ctypes as C
@@ -118,6 +123,12 @@ def bench_gravity(args, lib, gen):
     pairs = n * (n - 1)
     value = pairs * args.steps / elapsed
     kms, kname = ctx.time_force_kernel(20)
+    # the same run continued to 10k steps for the long-run energy drift
+    drift10k = None
+    if not args.no_secondary:
+        ctx.vv_run(dt, 10000 - args.steps - args.warmup)
+        ke2, _, pg2 = ctx.energies()
+        drift10k = abs((ke2 + pg2) - e0) / abs(e0)
     achieved = FLOP_GRAV * pairs / (kms * 1e-3) / 1e12
     sample_pairs = len(targets) * (n - 1)
     cpu_rate = sample_pairs / cpu_s
@@ -297,19 +308,29 @@ def bench_batch(args, lib, gen):
             "ms_per_step": 1e3 * el / steps, "atom_steps_per_s": sum(natoms) * steps / el,
             "pairs_per_step": int(pairs.sum()),
             "pair_interactions_per_s": float(pairs.sum()) * steps / el,
-            "temperature_rel_error_median": float(np.median(np.abs(T / Tt - 1.0))),
+            "temperature_rel_error_median": {
+                th: float(np.median(np.abs(T / Tt - 1.0)[[r.thermostat == th for r in reps]]))
+                for th in ("langevin", "andersen")},
             "cpu_baseline": {"steps_per_s": 1.0 / cpu_step_s, "cores": 1, "kind": "port",
                              "sample": f"FP64 oracle all-pairs RHS of 8 replicas ({cpu_s:.2f} s), "
                                        f"scaled by sum N^2 to all 512"},
             "gpu_vs_cpu_steps": (steps / el) * cpu_step_s}
 
 
-def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000):
-    """Learned 8-dt stride on LJ N=32,000 (FCC 20^3, rho* 0.8442, T* 0.9, NVE): training
-    set from `ntraj` plain trajectories of `traj_steps` steps (256 atoms per 8-step frame),
-    Adam training, then 10k dt-steps of simulated time with the learned stride (energy
-    monitor every 25 strided steps, fallback at 1e-4) against plain velocity Verlet."""
-    dt, stride = 0.005, 8
+def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
+                  sweep=(4, 2), sweep_traj=20):
+    """Learned stride on LJ N=32,000 (FCC 20^3, rho* 0.8442, T* 0.9, NVE), dt = 0.005.
+
+    For the configured stride 8: a training set from `ntraj` plain trajectories of
+    `traj_steps` steps (256 atoms per strided frame) plus held-out trajectories, Adam
+    training, then 10k dt-steps of simulated time with the learned stride (energy monitor
+    every 25 strided steps, fallback to plain velocity Verlet at a relative deviation of
+    1e-4), timed against plain velocity Verlet at dt from the same state.  The same is done
+    for the strides in `sweep` with `sweep_traj` trajectories each, and every stride is
+    also run with a zero network (the strided step without the learned correction)."""
+    dt = 0.005
+    nsteps_dt = 10000
+    sim_time = nsteps_dt * dt
 
     def ctx_for(seed):
         s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=seed)
@@ -321,58 +342,71 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000):
         ctx.set_state(s.u, s.v)
         return ctx, s
 
-    model = lib.Model(1)
-    heldm = lib.Model(1)
-    nheld = max(2, ntraj // 20)
-    t0 = time.perf_counter()
-    gctx, _ = ctx_for(0)
-    for tr in range(ntraj + nheld):
-        s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=1000 + tr)
-        gctx.set_state(s.u, s.v)
-        gctx.learn_collect(model if tr < ntraj else heldm, dt, stride, traj_steps // stride, 256,
-                           seed=tr)
-    t_collect = time.perf_counter() - t0
-    nsamp = model.num_samples
-    model.fit_normalization()
-    t0 = time.perf_counter()
-    losses = model.train(iters, batch=8192, lr=1e-3, seed=7)
-    t_train = time.perf_counter() - t0
-    # the plain integrator judges the strided step on trajectories never trained on
-    heldm.set(*model.get())
-    mse, base = heldm.evaluate(0, heldm.num_samples)
-    # evaluation: 10k dt-steps of simulated time from a fresh seeded state
+    def train(stride, ntr):
+        model = lib.Model(1)
+        heldm = lib.Model(1)
+        nheld = max(2, ntr // 20)
+        t0 = time.perf_counter()
+        gctx, _ = ctx_for(0)
+        for tr in range(ntr + nheld):
+            s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=1000 + tr)
+            gctx.set_state(s.u, s.v)
+            gctx.learn_collect(model if tr < ntr else heldm, dt, stride, traj_steps // stride,
+                               256, seed=tr)
+        t_collect = time.perf_counter() - t0
+        model.fit_normalization()
+        t0 = time.perf_counter()
+        losses = model.train(iters, batch=8192, lr=1e-3, seed=7)
+        t_train = time.perf_counter() - t0
+        # the plain integrator judges the strided step on trajectories never trained on
+        heldm.set(*model.get())
+        mse, base = heldm.evaluate(0, heldm.num_samples)
+        info = {"trajectories": ntr, "heldout_trajectories": nheld, "steps_each": traj_steps,
+                "samples": model.num_samples, "collect_s": t_collect, "iters": iters,
+                "batch": 8192, "train_s": t_train,
+                "final_loss": float(np.mean(losses[-50:])),
+                "heldout_mse_over_uncorrected": [float(x) for x in mse / base]}
+        return model, info
+
+    def run(model, stride):
+        ctx, _ = ctx_for(4242)
+        t0 = time.perf_counter()
+        rep = ctx.learned_run(model, dt, stride, nsteps_dt // stride, drift_tol=1e-4,
+                              check_every=25)
+        el = time.perf_counter() - t0
+        u, _ = ctx.get_state()
+        return {"sim_time_per_s": sim_time / el, "wall_s": el, "report": rep,
+                "energy_drift_rel_10k": abs(rep["E"] - rep["E0"]) / abs(rep["E0"]),
+                "fraction_of_time_strided": rep["strided_steps"] * stride / nsteps_dt}, u
+
+    # plain velocity Verlet at dt: the reference point and the judge
     plain, s = ctx_for(4242)
     ke0, pe0 = plain.learn_energy()
     plain.vv_run(dt, 20)
     plain.set_state(s.u, s.v)
     t0 = time.perf_counter()
-    plain.vv_run(dt, 10000)
+    plain.vv_run(dt, nsteps_dt)
     t_plain = time.perf_counter() - t0
     ke1, pe1 = plain.learn_energy()
     up, _ = plain.get_state()
-    learned, _ = ctx_for(4242)
-    t0 = time.perf_counter()
-    rep = learned.learned_run(model, dt, stride, 10000 // stride, drift_tol=1e-4, check_every=25)
-    t_learned = time.perf_counter() - t0
-    ul, _ = learned.get_state()
-    rs, g_plain = plain.rdf(up[None], 200) if hasattr(plain, "rdf") else (None, None)
-    _, g_learned = learned.rdf(ul[None], 200) if hasattr(learned, "rdf") else (None, None)
-    sim_time = 10000 * dt
     e0 = ke0 + pe0
-    return {"n": 32000, "dt": dt, "stride": stride,
-            "training": {"trajectories": ntraj, "steps_each": traj_steps, "samples": nsamp,
-                         "heldout_trajectories": nheld,
-                         "collect_s": t_collect, "iters": iters, "batch": 8192,
-                         "train_s": t_train, "final_loss": float(np.mean(losses[-50:])),
-                         "heldout_mse_over_uncorrected": (mse / base).tolist()},
-            "plain": {"sim_time_per_s": sim_time / t_plain, "steps_per_s": 10000 / t_plain,
-                      "energy_drift_rel_10k": abs((ke1 + pe1) - e0) / abs(e0)},
-            "learned": {"sim_time_per_s": sim_time / t_learned, "report": rep,
-                        "energy_drift_rel_10k": abs(rep["E"] - rep["E0"]) / abs(rep["E0"]),
-                        "fraction_of_time_strided": rep["strided_steps"] * stride / 10000.0},
-            "speedup_sim_time": t_plain / t_learned,
-            "rdf_max_abs_diff": (float(np.abs(np.asarray(g_plain) - np.asarray(g_learned)).max())
-                                 if g_plain is not None else None)}
+    out = {"n": 32000, "dt": dt, "simulated_time": sim_time,
+           "plain": {"sim_time_per_s": sim_time / t_plain, "steps_per_s": nsteps_dt / t_plain,
+                     "energy_drift_rel_10k": abs((ke1 + pe1) - e0) / abs(e0)},
+           "strides": {}}
+    zero = lib.Model(0)
+    zero.set(np.zeros(lib.MLP_PARAMS), np.concatenate([np.zeros(35), np.ones(35), np.ones(6)]))
+    rs, g_plain = plain.rdf(up[None], 200)
+    for stride in (8,) + tuple(sweep):
+        model, info = train(stride, ntraj if stride == 8 else sweep_traj)
+        learned, ul = run(model, stride)
+        _, g_l = plain.rdf(ul[None], 200)
+        learned["rdf_max_abs_diff_vs_plain"] = float(np.abs(g_l - g_plain).max())
+        without, _ = run(zero, stride)
+        out["strides"][str(stride)] = {
+            "training": info, "learned": learned, "without_network": without,
+            "speedup_sim_time_vs_plain": learned["sim_time_per_s"] / out["plain"]["sim_time_per_s"]}
+    return out
 
 
 def profile_kernels(lib, gen):

@@ -292,9 +292,11 @@ int nbx_learn_collect(nbx_ctx* ctx, nbx_model* model, double dt, int32_t stride,
 /* nsteps strided steps.  Every check_every strided steps the total (shifted) energy is
  * compared with the run's initial value; when |E - E0| / |E0| exceeds drift_tol the
  * state returns to the start of that block and plain velocity Verlet at dt runs the
- * remaining steps.  report[8]: strided steps taken, plain steps taken, tripped (0/1),
- * strided steps before the trip (-1: none), E0, final E, largest relative deviation,
- * Verlet-list builds. */
+ * remaining steps.  report[9]: strided steps taken, plain steps taken, tripped (0/1),
+ * strided steps before the trip (-1: none), E0, final E, largest relative deviation of
+ * the accepted blocks and the final state, Verlet-list builds during the strided blocks,
+ * the relative deviation that tripped the monitor (-1: list overflow or NaN; 0: none).  A list overflow inside a strided block
+ * (atoms piling up after an unstable step) trips the monitor like an energy excursion. */
 int nbx_learned_run(nbx_ctx* ctx, const nbx_model* model, double dt, int32_t stride,
                     int64_t nsteps, double drift_tol, int32_t check_every, double* report);
 

@@ -783,7 +783,8 @@ int mx_learn_collect(nbx_ctx* c, nbx_model* m, double dt, int stride, int64_t nf
 
 // report: [0] strided steps taken, [1] plain steps taken after a trip, [2] tripped (0/1),
 // [3] strided steps done before the trip, [4] E0, [5] final E, [6] largest relative
-// deviation seen by the monitor, [7] list builds during the run
+// deviation of the accepted blocks and the final state, [7] list builds in strided
+// blocks, [8] the relative deviation that tripped the monitor (-1: list overflow or NaN)
 int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_t nbig,
                    double tol, int check_every, double* report) {
     mx_plan p;
@@ -857,8 +858,9 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
 
     int64_t done = 0, plain = 0, trip_at = -1;
     int tripped = 0;
-    double maxdev = 0.0, E = E0;
-    double grow = 1.0;
+    double maxdev = 0.0, E = E0, trip_dev = 0.0;
+    int64_t builds = 0;
+    int builds_prev = builds0;
     const int64_t every = std::max(1, check_every);
     while (done < nbig) {
         const int64_t msteps = std::min<int64_t>(every, nbig - done);
@@ -873,31 +875,36 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
                                           hipMemcpyDeviceToDevice, s));
             return NBX_OK;
         };
-        for (int attempt = 0;; ++attempt) {
-            if ((rc = run_chunk(msteps))) return rc;
-            if ((rc = read_flags(c, hflags))) return rc;
-            if (!hflags[1]) break;
-            if (attempt > 8) return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow replay did not converge");
-            if ((rc = restore())) return rc;
-            if ((rc = grow_lists(c, p, grow))) return rc;
-            if ((rc = prologue(c, p))) return rc;
-        }
+        if ((rc = run_chunk(msteps))) return rc;
+        if ((rc = read_flags(c, hflags))) return rc;
+        builds += hflags[2] - builds_prev;
+        builds_prev = hflags[2];
+        // a list overflow inside a strided block means atoms piled up (the strided step
+        // went unstable): the monitor trips as it does on an energy excursion
+        bool bad = hflags[1] != 0;
         double ke, pe;
-        if ((rc = list_energy(c, (int)(msteps & 1), lj, epart, &ke, &pe))) return rc;
-        E = ke + pe;
-        const double dev = std::fabs(E - E0) / std::fabs(E0);
-        const bool bad = !(dev <= tol);  // also catches NaN
+        trip_dev = -1.0;
         if (!bad) {
-            maxdev = std::max(maxdev, dev);
-            done += msteps;
-            c->steps_taken += msteps * stride;
-            continue;
+            if ((rc = list_energy(c, (int)(msteps & 1), lj, epart, &ke, &pe))) return rc;
+            E = ke + pe;
+            const double dev = std::fabs(E - E0) / std::fabs(E0);
+            trip_dev = std::isfinite(dev) ? dev : -1.0;
+            bad = !(dev <= tol);  // also catches NaN
+            if (!bad) {
+                maxdev = std::max(maxdev, dev);
+                done += msteps;
+                c->steps_taken += msteps * stride;
+                continue;
+            }
         }
         // the monitor tripped: back to the chunk's start, plain velocity Verlet from there
         tripped = 1;
         trip_at = done;
         if ((rc = restore())) return rc;
-        c->accel_valid = true;
+        // lists built during the failed block are not trusted: fresh lists and forces
+        mx_launch_fill_int(s, c->nl.d_flags + 1, 1, 0);
+        c->nl.ready = false;
+        c->accel_valid = false;
         plain = (nbig - done) * stride;
         int64_t nf = 0;
         if ((rc = mx_vv(c, dt, plain, 0, 0, nullptr, nullptr, &nf))) return rc;
@@ -907,7 +914,6 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         maxdev = std::max(maxdev, std::fabs(E - E0) / std::fabs(E0));
         break;
     }
-    if ((rc = read_flags(c, hflags))) return rc;
     if (report) {
         report[0] = (double)done;
         report[1] = (double)plain;
@@ -916,7 +922,8 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         report[4] = E0;
         report[5] = E;
         report[6] = maxdev;
-        report[7] = (double)(hflags[2] - builds0);
+        report[7] = (double)builds;
+        report[8] = tripped ? trip_dev : 0.0;
     }
     return NBX_OK;
 }

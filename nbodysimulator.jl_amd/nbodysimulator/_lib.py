@@ -328,13 +328,13 @@ class Context:
                                                 int(nframes), int(atoms_per_frame), int(seed)))
 
     def learned_run(self, model, dt, stride, nsteps, drift_tol=1e-4, check_every=25) -> dict:
-        r = np.zeros(8)
+        r = np.zeros(9)
         self._check(self._lib.nbx_learned_run(self._h, model._h, float(dt), int(stride),
                                               int(nsteps), float(drift_tol), int(check_every),
                                               dptr(r)))
         return dict(strided_steps=int(r[0]), plain_steps=int(r[1]), tripped=bool(r[2]),
                     strided_before_trip=int(r[3]), E0=r[4], E=r[5], max_rel_dev=r[6],
-                    list_builds=int(r[7]))
+                    list_builds=int(r[7]), trip_rel_dev=r[8])
 
     def stats(self) -> dict:
         out = (C.c_int64 * 5)()
