@@ -427,12 +427,35 @@ def profile_kernels(lib, gen):
     return 0
 
 
+def profile_learned(lib, gen, stride=2, nsteps=200):
+    """Strided steps of LJ N=32,000 with a zero network (every kernel of the strided step
+    runs; the corrections are zero), for a kernel trace."""
+    s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=4242)
+    ctx = lib.Context(s.u.shape[0], lib.SYS_POTENTIAL, s.masses)
+    ctx.set_boundary(lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+    ctx.add_lennard_jones(1.0, 1.0, 2.5)
+    ctx.set_kb(1.0)
+    ctx.set_precision(lib.PREC_MIXED)
+    ctx.set_state(s.u, s.v)
+    zero = lib.Model(0)
+    zero.set(np.zeros(lib.MLP_PARAMS), np.concatenate([np.zeros(35), np.ones(35), np.ones(6)]))
+    t0 = time.perf_counter()
+    rep = ctx.learned_run(zero, 0.005, stride, nsteps, drift_tol=1.0, check_every=25)
+    el = time.perf_counter() - t0
+    print(json.dumps({"stride": stride, "strided_steps": nsteps,
+                      "us_per_strided_step": 1e6 * el / nsteps, "report": rep}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--profile-learned", action="store_true",
+                    help="only run 200 strided steps (stride 2, zero network) of LJ N=32000 "
+                         "for a kernel trace of the strided step")
     ap.add_argument("--learn-traj", type=int, default=200,
                     help="training trajectories of the learned-stride line")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -451,6 +474,8 @@ def main():
         return 1
     if args.profile_kernels:
         return profile_kernels(lib, gen)
+    if args.profile_learned:
+        return profile_learned(lib, gen)
     log("[bench] gravity N=65536")
     g = bench_gravity(args, lib, gen)
     res = {

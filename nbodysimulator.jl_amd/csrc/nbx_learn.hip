@@ -26,7 +26,11 @@ __device__ __forceinline__ f32x16 lr_mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float lr_sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+// logistic function from the hardware 2^x and reciprocal (relative error ~1e-6, far
+// below the BF16 rounding that follows it)
+__device__ __forceinline__ float lr_sigmoid(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
 
 // One 128-output layer with its epilogue, one M tile at a time (16 accumulators live):
 // acc = sum_kt W(mt, kt) Bin(kt); then per register r (unit u = 32 mt + drow(h, r)):
@@ -383,10 +387,11 @@ __global__ void __launch_bounds__(256) k_lr_features(lr_feat_params fp) {
         if (r2 < 6.25) {
             const float r = sqrtf((float)r2);
             const float sw = r < 2.0f ? 1.0f : 0.5f * (1.0f + cosf(3.14159265358979f * (r - 2.0f) * 2.0f));
+            // exp(-50 e^2) = 2^(-50 log2(e) e^2)
 #pragma unroll
             for (int b = 0; b < LR_NBINS; ++b) {
                 const float e = r - lr_centre(b);
-                g[b] += sw * expf(-50.0f * e * e);
+                g[b] += sw * __builtin_amdgcn_exp2f(-72.1347520444482f * e * e);
             }
         }
     }
@@ -535,10 +540,23 @@ __global__ void __launch_bounds__(256) k_lr_drift(const double* __restrict__ par
                                                   const float* __restrict__ dlab, double D,
                                                   mx_io io, int n) {
     __shared__ double smean[6];
-    if (threadIdx.x < 6) {
-        double acc = 0.0;
-        for (int q = 0; q < nparts; ++q) acc += part[(size_t)q * 6 + threadIdx.x];
-        smean[threadIdx.x] = acc / (double)n;
+    if (threadIdx.x < 64) {
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int q = threadIdx.x; q < nparts; q += 64) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] += part[(size_t)q * 6 + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off);
+        }
+        if (threadIdx.x < 6) {
+            double m = acc[0];
+#pragma unroll
+            for (int c = 1; c < 6; ++c) if ((int)threadIdx.x == c) m = acc[c];
+            smean[threadIdx.x] = m / (double)n;
+        }
     }
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -671,12 +689,25 @@ __global__ void __launch_bounds__(256) k_lr_energy(lr_energy_params ep) {
     }
 }
 
-__global__ void k_lr_energy_final(const double* part, int nparts, double* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(256) k_lr_energy_final(const double* part, int nparts,
+                                                         double* out) {
+    __shared__ double sk[256], sp[256];
     double k = 0.0, p = 0.0;
-    for (int q = 0; q < nparts; ++q) { k += part[2 * q]; p += part[2 * q + 1]; }
-    out[0] = k;
-    out[1] = p;
+    for (int q = threadIdx.x; q < nparts; q += 256) { k += part[2 * q]; p += part[2 * q + 1]; }
+    sk[threadIdx.x] = k;
+    sp[threadIdx.x] = p;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sk[threadIdx.x] += sk[threadIdx.x + off];
+            sp[threadIdx.x] += sp[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sk[0];
+        out[1] = sp[0];
+    }
 }
 
 int lr_energy_blocks(int64_t n, int lanes) { return (int)((n * lanes + 255) / 256); }
@@ -691,5 +722,5 @@ void lr_launch_energy(hipStream_t s, int lanes, const lr_energy_params& ep, doub
     case 16: hipLaunchKernelGGL(k_lr_energy<16>, dim3(blocks), dim3(256), 0, s, ep); break;
     default: hipLaunchKernelGGL(k_lr_energy<8>, dim3(blocks), dim3(256), 0, s, ep); break;
     }
-    hipLaunchKernelGGL(k_lr_energy_final, dim3(1), dim3(64), 0, s, ep.part, blocks, out);
+    hipLaunchKernelGGL(k_lr_energy_final, dim3(1), dim3(256), 0, s, ep.part, blocks, out);
 }
