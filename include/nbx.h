@@ -96,6 +96,12 @@ int nbx_add_electrostatic(nbx_ctx* ctx, double k, double R);
 int nbx_add_reaction_field(nbx_ctx* ctx, double k, double R, double eps_rf);
 /* GravitationalParameters(G): src/basic_potentials.jl:109-115, :306-331 */
 int nbx_add_gravitational(nbx_ctx* ctx, double G);
+/* Magnetic moments of MagneticParticle bodies (src/bodies.jl:102-107): mm is 3 x n
+ * column-major (mm of body i in column i). */
+int nbx_set_moments(nbx_ctx* ctx, const double* mm);
+/* MagnetostaticParameters(mu_4pi): src/basic_potentials.jl:171-205, :333-365 (dipole-dipole,
+ * all pairs, no boundary conditions, as the reference).  Needs nbx_set_moments first. */
+int nbx_add_magnetostatic(nbx_ctx* ctx, double mu_4pi);
 /* SPCFwParameters(rOH, aHOH, kb, ka): src/basic_potentials.jl:233-238, :367-433 */
 int nbx_set_spcfw(nbx_ctx* ctx, double rOH, double aHOH, double kbond, double kangle);
 /* Thermostat (NBX_THERM_*), target T and its parameter (nu, tau or gamma). */

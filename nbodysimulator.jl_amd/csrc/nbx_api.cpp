@@ -43,6 +43,10 @@ static void release(nbx_ctx* c) {
     free_ptr(c->d_x); free_ptr(c->d_v); free_ptr(c->d_a); free_ptr(c->d_vh);
     free_ptr(c->d_tmp); free_ptr(c->d_ms); free_ptr(c->d_qs); free_ptr(c->d_red);
     free_ptr(c->d_scal);
+    free_ptr(c->d_mm);
+    free_ptr(c->d_mm4);
+    c->d_mm = nullptr;
+    c->d_mm4 = nullptr;
     c->d_x = c->d_v = c->d_a = c->d_vh = c->d_tmp = c->d_ms = c->d_qs = nullptr;
     c->d_red = c->d_scal = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -226,6 +230,43 @@ extern "C" int nbx_add_gravitational(nbx_ctx* c, double G) {
     return add_term(c, NBX_TERM_GRAV, G, 0.0, 0.0);
 }
 
+static int upload_aos(nbx_ctx* c, const double* host, double* d_soa);
+static bool all_finite(const double* p, int64_t count);
+
+extern "C" int nbx_add_magnetostatic(nbx_ctx* c, double mu_4pi) {
+    if (!c) return NBX_ERR_INVALID;
+    if (c->sys != NBX_SYS_POTENTIAL)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "magnetic dipoles apply to potential systems");
+    if (!c->mm_set) return nbx_fail(c, NBX_ERR_INVALID, "magnetostatics needs moments (nbx_set_moments)");
+    if (!std::isfinite(mu_4pi)) return nbx_fail(c, NBX_ERR_INVALID, "mu/4pi must be finite");
+    return add_term(c, NBX_TERM_MAG, mu_4pi, 0.0, 0.0);
+}
+
+extern "C" int nbx_set_moments(nbx_ctx* c, const double* mm) {
+    if (!c || !mm) return NBX_ERR_INVALID;
+    const int64_t n = c->n;
+    if (!all_finite(mm, 3 * n)) return nbx_fail(c, NBX_ERR_NONFINITE, "moments hold NaN or Inf");
+    if (!c->d_mm) NBX_HIP(c, hipMalloc((void**)&c->d_mm, sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1)));
+    if (!c->d_mm4) NBX_HIP(c, hipMalloc(&c->d_mm4, sizeof(float) * 4 * (size_t)std::max<int64_t>(n, 1)));
+    if (n > 0) {
+        int rc = upload_aos(c, mm, c->d_mm);
+        if (rc) return rc;
+        std::vector<float> f4((size_t)4 * n);
+        for (int64_t i = 0; i < n; ++i) {
+            f4[4 * i] = (float)mm[3 * i];
+            f4[4 * i + 1] = (float)mm[3 * i + 1];
+            f4[4 * i + 2] = (float)mm[3 * i + 2];
+            f4[4 * i + 3] = 0.0f;
+        }
+        NBX_HIP(c, hipMemcpyAsync(c->d_mm4, f4.data(), sizeof(float) * f4.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+        NBX_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->mm_set = true;
+    invalidate(c);
+    return NBX_OK;
+}
+
 extern "C" int nbx_set_spcfw(nbx_ctx* c, double rOH, double aHOH, double kbond, double kangle) {
     if (!c) return NBX_ERR_INVALID;
     if (c->sys != NBX_SYS_WATER)
@@ -353,6 +394,9 @@ static void ref_term(nbx_ctx* c, const nbx_term& t, const double* x, double* dv,
         break;
     case NBX_TERM_GRAV:
         ref64_grav(s, x, dv, c->d_ms, c->n, ib, ie, t.p[0]);
+        break;
+    case NBX_TERM_MAG:
+        ref64_mag(s, x, dv, c->d_ms, c->d_mm, c->n, ib, ie, t.p[0]);
         break;
     default:
         break;

@@ -141,6 +141,71 @@ int mx_gravity_chunks(const nbx_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+// All-pairs magnetic dipoles (src/basic_potentials.jl:333-365) in mixed precision: FP32
+// pair law on float4 positions and float4 moments staged 256 at a time in LDS, plain FP32
+// sums per tile, Kahan across tiles; the per-target sum divided by m_i goes to `part`
+// (FP64), and k_grav_finish applies 3 mu/4pi, the kick and the drift.  No boundary
+// conditions (as the reference); j == i skipped.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(GR_THREADS)
+k_mag_tiled(const float4* __restrict__ pos, const float4* __restrict__ mm, int n,
+            double* __restrict__ part) {
+    __shared__ float4 tp[GR_TILE];
+    __shared__ float4 tm[GR_TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * GR_THREADS + tid;
+    const float4 pi = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 1.f);
+    const float4 mi = i < n ? mm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    nbx_kahan ax, ay, az;
+    ax.init(); ay.init(); az.init();
+    for (int base = 0; base < n; base += GR_TILE) {
+        const int j = base + tid;
+        tp[tid] = j < n ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        tm[tid] = j < n ? mm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const int lim = (n - base) < GR_TILE ? (n - base) : GR_TILE;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int q = 0; q < lim; ++q) {
+            if (base + q == i) continue;
+            const float4 pj = tp[q];
+            const float4 mj = tm[q];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float d = rx * rx + ry * ry + rz * rz;
+            const float inv = __builtin_amdgcn_rsqf(d);
+            const float inv2 = inv * inv;
+            const float inv4 = inv2 * inv2;
+            const float r1 = rx * inv, r2 = ry * inv, r3 = rz * inv;
+            const float mir = mi.x * r1 + mi.y * r2 + mi.z * r3;
+            const float mij = mj.x * r1 + mj.y * r2 + mj.z * r3;
+            const float dmm = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
+            const float f5 = 5.0f * mir * mij;
+            sx += (mi.x * mij + mj.x * mir + r1 * (dmm - f5)) * inv4;
+            sy += (mi.y * mij + mj.y * mir + r2 * (dmm - f5)) * inv4;
+            sz += (mi.z * mij + mj.z * mir + r3 * (dmm - f5)) * inv4;
+        }
+        ax.add(sx); ay.add(sy); az.add(sz);
+        __syncthreads();
+    }
+    if (i < n) {
+        const double im = 1.0 / (double)pi.w;
+        part[i] = ax.value() * im;
+        part[n + i] = ay.value() * im;
+        part[2 * n + i] = az.value() * im;
+    }
+}
+
+void mx_launch_magnetic(hipStream_t s, const nbx_ctx* c, const void* pos, double mu_4pi,
+                        const mx_kick_params& kp, const mx_io& io, int force_only_kernel) {
+    const int n = (int)c->n;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mag_tiled, dim3(nbx_blocks(n, GR_THREADS)), dim3(GR_THREADS), 0, s,
+                       (const float4*)pos, (const float4*)c->d_mm4, n, c->d_part);
+    if (force_only_kernel) return;
+    hipLaunchKernelGGL(k_grav_finish, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, c->d_part, 1, n,
+                       3.0 * mu_4pi, kp, io);
+}
+
+// ---------------------------------------------------------------------------
 // Plain device-to-device copy: the calibration kernel for HBM byte counters and the
 // achievable-bandwidth probe.  Reads `n16` 16-byte words from src, writes them to dst.
 // ---------------------------------------------------------------------------

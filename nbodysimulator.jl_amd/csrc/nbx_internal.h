@@ -10,11 +10,11 @@
 #include "nbx.h"
 #include "nbx_device.h"
 
-enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4 };
+enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4, NBX_TERM_MAG = 5 };
 
 struct nbx_term {
     int kind;
-    double p[4];  // LJ: eps sigma R; ELEC: k R; GRAV: G; RF: k R eps_rf
+    double p[4];  // LJ: eps sigma R; ELEC: k R; GRAV: G; RF: k R eps_rf; MAG: mu/4pi
 };
 
 // Mixed-path neighbour machinery (cell list + Verlet list)
@@ -81,6 +81,9 @@ struct nbx_ctx {
     double* d_tmp = nullptr;  // scratch 3n (host transfers, rhs)
     double* d_ms = nullptr;
     double* d_qs = nullptr;
+    double* d_mm = nullptr;   // magnetic moments (SoA 3n), when set
+    void* d_mm4 = nullptr;    // the same as float4 (mx, my, mz, 0) for the mixed path
+    bool mm_set = false;
     double* d_red = nullptr;  // reduction scratch (n + 64 doubles)
     double* d_scal = nullptr; // device scalars: [0] temperature for Berendsen
     void* d_pos = nullptr;    // mixed path: double4 wrapped positions (+charge) or float4 (+mass)
@@ -131,6 +134,8 @@ void ref64_elec(hipStream_t s, const double* x, double* dv, const double* ms,
 void ref64_rf(hipStream_t s, const double* x, double* dv, const double* ms, const double* qs,
               int64_t n, int64_t ib, int64_t ie, int water, double k, double R,
               double eps_rf, const nbx_bcv& bc);
+void ref64_mag(hipStream_t s, const double* x, double* dv, const double* ms, const double* mm,
+               int64_t n, int64_t ib, int64_t ie, double mu_4pi);
 void ref64_grav(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
                 int64_t ib, int64_t ie, double G);
 void ref64_bond(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,

@@ -21,13 +21,14 @@ int mx_gravity_chunks(const nbx_ctx* c);
 
 namespace {
 
-enum { PLAN_NONBONDED = 1, PLAN_WATER = 2, PLAN_GRAVITY = 3 };
+enum { PLAN_NONBONDED = 1, PLAN_WATER = 2, PLAN_GRAVITY = 3, PLAN_MAGNETIC = 4 };
 
 struct mx_plan {
     int kind = 0;
     mx_nb_launch nb{};
     mx_pair_params pp{};
     double G = 0.0;
+    double mu = 0.0;      // mu / 4 pi (magnetic dipoles)
     double cutoff = 0.0;  // largest finite cutoff (list mode)
 };
 
@@ -45,9 +46,10 @@ bool make_plan(const nbx_ctx* c, mx_plan& p, std::string* why) {
         p.kind = PLAN_WATER;
         p.nb.water = 1;
     } else {
-        int ngrav = 0;
+        int ngrav = 0, nmag = 0;
         for (const nbx_term& t : c->terms) {
             if (t.kind == NBX_TERM_GRAV) { ++ngrav; p.G = t.p[0]; }
+            else if (t.kind == NBX_TERM_MAG) { ++nmag; p.mu = t.p[0]; }
             else if (t.kind == NBX_TERM_LJ) {
                 if (lj) { if (why) *why = "mixed path: one Lennard-Jones term at most"; return false; }
                 lj = &t;
@@ -55,6 +57,14 @@ bool make_plan(const nbx_ctx* c, mx_plan& p, std::string* why) {
                 if (el) { if (why) *why = "mixed path: one electrostatic term at most"; return false; }
                 el = &t;
             }
+        }
+        if (nmag) {
+            if (nmag > 1 || ngrav || lj || el) {
+                if (why) *why = "mixed path: magnetic dipoles cannot be combined with other potentials";
+                return false;
+            }
+            p.kind = PLAN_MAGNETIC;
+            return true;
         }
         if (ngrav) {
             if (ngrav > 1 || lj || el) {
@@ -160,8 +170,10 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     return NBX_OK;
 }
 
+bool all_pairs_tiled(const mx_plan& p) { return p.kind == PLAN_GRAVITY || p.kind == PLAN_MAGNETIC; }
+
 int pos_kind(const mx_plan& p) {
-    if (p.kind == PLAN_GRAVITY) return 2;
+    if (all_pairs_tiled(p)) return 2;
     return p.nb.mode_list ? 1 : 0;
 }
 
@@ -255,6 +267,8 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
     const mx_io io = make_io(c, p, par, do_drift);
     if (p.kind == PLAN_GRAVITY) {
         mx_launch_gravity(s, c, pbuf(c, par), p.G, kp, io, 0);
+    } else if (p.kind == PLAN_MAGNETIC) {
+        mx_launch_magnetic(s, c, pbuf(c, par), p.mu, kp, io, 0);
     } else {
         mx_nb_launch nb = p.nb;
         nb.lanes = c->nl.lanes;
@@ -294,8 +308,8 @@ static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
     if (!c->d_x2)
         NBX_HIP(c, hipMalloc((void**)&c->d_x2, sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1)));
     if (!c->d_step) NBX_HIP(c, hipMalloc((void**)&c->d_step, sizeof(int64_t)));
-    if (p.kind == PLAN_GRAVITY) {
-        const int chunks = mx_gravity_chunks(c);
+    if (all_pairs_tiled(p)) {
+        const int chunks = p.kind == PLAN_GRAVITY ? mx_gravity_chunks(c) : 1;
         if (!c->d_part || c->grav_chunks != chunks) {
             if (c->d_part) (void)hipFree(c->d_part);
             c->d_part = nullptr;
@@ -512,6 +526,7 @@ int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
     nb.lanes = c->nl.lanes;
     auto launch = [&]() {
         if (p.kind == PLAN_GRAVITY) mx_launch_gravity(s, c, c->d_pos, p.G, kp, io, 1);
+        else if (p.kind == PLAN_MAGNETIC) mx_launch_magnetic(s, c, c->d_pos, p.mu, kp, io, 1);
         else mx_launch_nonbonded(s, nb, c, c->d_pos, p.pp, kp, io, nullptr);
     };
     launch();  // warm
@@ -529,6 +544,7 @@ int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
     *avg_ms = (double)ms / iters;
     if (name) {
         if (p.kind == PLAN_GRAVITY) *name = "k_grav_tiled";
+        else if (p.kind == PLAN_MAGNETIC) *name = "k_mag_tiled";
         else if (p.nb.mode_list) *name = "k_nb_list";
         else *name = "k_nb_allpairs";
     }
@@ -543,7 +559,7 @@ int mx_count_pairs(nbx_ctx* c, int64_t* pairs) {
     mx_plan p;
     std::string why;
     if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
-    if (p.kind == PLAN_GRAVITY) {
+    if (all_pairs_tiled(p)) {
         *pairs = c->n * (c->n - 1);
         return NBX_OK;
     }

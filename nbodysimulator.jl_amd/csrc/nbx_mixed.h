@@ -207,6 +207,9 @@ void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& 
                            unsigned long long* d_count);
 void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double G,
                        const mx_kick_params& kp, const mx_io& io, int force_only_kernel);
+// all-pairs magnetic dipoles: force into c->d_part (one chunk), then k_grav_finish
+void mx_launch_magnetic(hipStream_t s, const nbx_ctx* c, const void* pos, double mu_4pi,
+                        const mx_kick_params& kp, const mx_io& io, int force_only_kernel);
 void mx_launch_step_advance(hipStream_t s, int64_t* step_base, int64_t delta, int set);
 void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value);
 // bins of P (gated by *req; the bins were cleared by the preceding build kernel)

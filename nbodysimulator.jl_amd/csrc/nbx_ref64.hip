@@ -235,14 +235,77 @@ void ref64_grav(hipStream_t s, const double* x, double* dv, const double* ms, in
                        x, dv, ms, n, ib, ie, G);
 }
 
-// ---------------------------------------------------------------------------
-// SPC/Fw bonded terms: src/basic_potentials.jl:367-433
-// ---------------------------------------------------------------------------
-
+// sqrt((x^2 + y^2) + z^2)
 __device__ __forceinline__ double ref_norm3(double x, double y, double z) {
     double s = x * x + y * y;
     return sqrt(s + z * z);
 }
+
+// ---------------------------------------------------------------------------
+// Magnetic dipoles: src/basic_potentials.jl:333-365 (no boundary conditions, as the
+// reference).  Per source j, in the reference's expression order:
+//   rij4 = dot(rij, rij)^2, r = rij / norm(rij), mir = mi.r, mij = mj.r,
+//   c_k = (((mi_k mij + mj_k mir) + r_k (mi.mj)) - ((5 r_k) mir) mij) / rij4
+// then dv += (3 mu_4pi / m_i) * sum_j c.  mm: moments SoA (component k at k*n + j).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(REF_TILE)
+k_ref_mag(const double* __restrict__ x, double* __restrict__ dv, const double* __restrict__ ms,
+          const double* __restrict__ mm, int64_t n, int64_t ib, int64_t ie, double mu_4pi) {
+    __shared__ double sx[REF_TILE], sy[REF_TILE], sz[REF_TILE];
+    __shared__ double smx[REF_TILE], smy[REF_TILE], smz[REF_TILE];
+    const int64_t i = ib + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < ie;
+    double xi = 0.0, yi = 0.0, zi = 0.0, mix = 0.0, miy = 0.0, miz = 0.0;
+    if (active) {
+        xi = x[i]; yi = x[n + i]; zi = x[2 * n + i];
+        mix = mm[i]; miy = mm[n + i]; miz = mm[2 * n + i];
+    }
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int64_t base = 0; base < n; base += REF_TILE) {
+        const int64_t j = base + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = x[j]; sy[threadIdx.x] = x[n + j]; sz[threadIdx.x] = x[2 * n + j];
+            smx[threadIdx.x] = mm[j]; smy[threadIdx.x] = mm[n + j]; smz[threadIdx.x] = mm[2 * n + j];
+        }
+        __syncthreads();
+        const int lim = (int)((n - base) < REF_TILE ? (n - base) : REF_TILE);
+        if (active) {
+            for (int q = 0; q < lim; ++q) {
+                if (base + q == i) continue;
+                const double rx = xi - sx[q], ry = yi - sy[q], rz = zi - sz[q];
+                const double d = (rx * rx + ry * ry) + rz * rz;
+                const double rij4 = d * d;
+                const double nr = ref_norm3(rx, ry, rz);
+                const double r1 = rx / nr, r2 = ry / nr, r3 = rz / nr;
+                const double mjx = smx[q], mjy = smy[q], mjz = smz[q];
+                const double mir = (mix * r1 + miy * r2) + miz * r3;
+                const double mij = (mjx * r1 + mjy * r2) + mjz * r3;
+                const double dmm = (mix * mjx + miy * mjy) + miz * mjz;
+                f1 += (((mix * mij + mjx * mir) + r1 * dmm) - ((5.0 * r1) * mir) * mij) / rij4;
+                f2 += (((miy * mij + mjy * mir) + r2 * dmm) - ((5.0 * r2) * mir) * mij) / rij4;
+                f3 += (((miz * mij + mjz * mir) + r3 * dmm) - ((5.0 * r3) * mir) * mij) / rij4;
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        const double coeff = 3.0 * mu_4pi / ms[i];
+        dv[i] += coeff * f1;
+        dv[n + i] += coeff * f2;
+        dv[2 * n + i] += coeff * f3;
+    }
+}
+
+void ref64_mag(hipStream_t s, const double* x, double* dv, const double* ms, const double* mm,
+               int64_t n, int64_t ib, int64_t ie, double mu_4pi) {
+    if (ie <= ib) return;
+    hipLaunchKernelGGL(k_ref_mag, dim3(nbx_blocks(ie - ib, REF_TILE)), dim3(REF_TILE), 0, s, x,
+                       dv, ms, mm, n, ib, ie, mu_4pi);
+}
+
+// ---------------------------------------------------------------------------
+// SPC/Fw bonded terms: src/basic_potentials.jl:367-433
+// ---------------------------------------------------------------------------
 
 __global__ void k_ref_bond(const double* __restrict__ x, double* __restrict__ dv,
                            const double* __restrict__ ms, int64_t n, int64_t ib, int64_t ie,

@@ -48,6 +48,8 @@ SIGNATURES = {
     "nbx_add_electrostatic": (C.c_int, [_P, _D, _D]),
     "nbx_add_reaction_field": (C.c_int, [_P, _D, _D, _D]),
     "nbx_add_gravitational": (C.c_int, [_P, _D]),
+    "nbx_set_moments": (C.c_int, [_P, _DP]),
+    "nbx_add_magnetostatic": (C.c_int, [_P, _D]),
     "nbx_set_spcfw": (C.c_int, [_P, _D, _D, _D, _D]),
     "nbx_set_thermostat": (C.c_int, [_P, _I32, _D, _D]),
     "nbx_set_skin": (C.c_int, [_P, _D]),
@@ -213,6 +215,14 @@ class Context:
 
     def add_gravitational(self, G):
         self._check(self._lib.nbx_add_gravitational(self._h, float(G)))
+
+    def set_moments(self, mm):
+        """Magnetic moments, (n, 3) (the memory layout of a Julia 3 x n matrix)."""
+        self._mm = as_c3n(mm, self.n)
+        self._check(self._lib.nbx_set_moments(self._h, dptr(self._mm)))
+
+    def add_magnetostatic(self, mu_4pi):
+        self._check(self._lib.nbx_add_magnetostatic(self._h, float(mu_4pi)))
 
     def set_spcfw(self, rOH, aHOH, kbond, kangle):
         self._check(self._lib.nbx_set_spcfw(self._h, float(rOH), float(aHOH), float(kbond),

@@ -16,7 +16,7 @@ from . import _lib
 from .types import (AndersenThermostat, BerendsenThermostat, ChargedParticle,
                     CubicPeriodicBoundaryConditions, ElectrostaticParameters,
                     GravitationalParameters, InfiniteBox, LangevinThermostat,
-                    LennardJonesParameters, MagnetostaticParameters, NBodySimulation,
+                    LennardJonesParameters, MagneticParticle, MagnetostaticParameters, NBodySimulation,
                     NoseHooverThermostat, NullThermostat, PeriodicBoundaryConditions,
                     PotentialNBodySystem, ReactionFieldParameters, SPCFwParameters,
                     WaterSPCFw, gather_bodies_initial_coordinates, get_coordinate_vector_count,
@@ -110,7 +110,7 @@ def _add_potential(ctx: _lib.Context, p):
     elif isinstance(p, GravitationalParameters):
         ctx.add_gravitational(p.G)
     elif isinstance(p, MagnetostaticParameters):
-        raise NotImplementedError("magnetostatic dipoles are not part of the GPU path")
+        ctx.add_magnetostatic(p.μ_4π)
     else:
         raise NotImplementedError(f"no GPU kernel for {type(p).__name__}; "
                                   "use a custom closure (get_accelerating_function)")
@@ -161,6 +161,8 @@ def build_context(simulation: NBodySimulation, precision: str = "fp64",
     else:
         ctx = _lib.Context(len(masses), _lib.SYS_POTENTIAL, masses, q)
         _set_bc(ctx, simulation.boundary_conditions)
+        if system.bodies and all(isinstance(b, MagneticParticle) for b in system.bodies):
+            ctx.set_moments(np.array([b.mm for b in system.bodies], dtype=np.float64))
         params = [only] if only is not None else list(system.potentials.values())
         for p in params:
             _add_potential(ctx, p)

@@ -313,6 +313,9 @@ static void apply_term(const orc_system* s, const orc_term* t, double* dvi, cons
     case ORC_TERM_GRAV:
         orc_grav_accel(dvi, u, i, s->n, s->ms, t->p[0]);
         break;
+    case ORC_TERM_MAG:
+        orc_mag_accel(dvi, u, i, s->n, s->ms, s->mm, t->p[0]);
+        break;
     default:
         break;
     }
@@ -688,6 +691,57 @@ void orc_rdf(const double* frames, int64_t nframes, int64_t n, const int64_t* in
         rs[bin - 1] = rl + dr / 2.0;
     }
     free(hist);
+}
+
+/* ---------------------------------------------------------------------------
+ * Magnetic dipoles: src/basic_potentials.jl:333-365
+ * ------------------------------------------------------------------------- */
+static double dot3s(const double* a, const double* b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+void orc_mag_accel(double* dv, const double* rs, int64_t i, int64_t n, const double* ms,
+                   const double* mm, double mu_4pi) {
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    const double* mi = mm + 3 * i;
+    const double* ri = rs + 3 * i;
+    for (int64_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double* mj = mm + 3 * j;
+        const double* rj = rs + 3 * j;
+        const double rij[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+        const double d = dot3s(rij, rij);
+        const double rij4 = d * d;                                   /* dot(rij, rij)^2 */
+        const double nr = norm3(rij);
+        const double r[3] = {rij[0] / nr, rij[1] / nr, rij[2] / nr};  /* rij / norm(rij) */
+        const double mir = dot3s(mi, r);
+        const double mij = dot3s(mj, r);
+        const double dmm = dot3s(mi, mj);
+        double c[3];
+        for (int k = 0; k < 3; ++k)
+            c[k] = (((mi[k] * mij + mj[k] * mir) + r[k] * dmm) - ((5.0 * r[k]) * mir) * mij) / rij4;
+        f1 += c[0];
+        f2 += c[1];
+        f3 += c[2];
+    }
+    const double coeff = 3.0 * mu_4pi / ms[i];
+    dv[0] += coeff * f1;
+    dv[1] += coeff * f2;
+    dv[2] += coeff * f3;
+}
+
+double orc_mag_potential(const double* rs, int64_t n, const double* mm, double mu_4pi) {
+    double e = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            const double rij[3] = {rs[3 * i] - rs[3 * j], rs[3 * i + 1] - rs[3 * j + 1],
+                                   rs[3 * i + 2] - rs[3 * j + 2]};
+            const double nr = norm3(rij);
+            const double r[3] = {rij[0] / nr, rij[1] / nr, rij[2] / nr};
+            e += (dot3s(mm + 3 * i, mm + 3 * j) -
+                  3.0 * dot3s(mm + 3 * i, r) * dot3s(mm + 3 * j, r)) / (nr * nr * nr);
+        }
+    return mu_4pi * e;
 }
 
 /* ---------------------------------------------------------------------------

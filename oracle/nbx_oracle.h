@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 enum { ORC_BC_INFINITE = 0, ORC_BC_CUBIC = 1, ORC_BC_PERIODIC = 2 };
-enum { ORC_TERM_LJ = 1, ORC_TERM_ELEC = 2, ORC_TERM_GRAV = 3, ORC_TERM_RF = 4 };
+enum { ORC_TERM_LJ = 1, ORC_TERM_ELEC = 2, ORC_TERM_GRAV = 3, ORC_TERM_RF = 4, ORC_TERM_MAG = 5 };
 enum { ORC_THERM_NONE = 0, ORC_THERM_ANDERSEN = 1, ORC_THERM_BERENDSEN = 2,
        ORC_THERM_LANGEVIN = 3 };
 
@@ -33,7 +33,7 @@ typedef struct {
 typedef struct {
     int32_t kind;     /* ORC_TERM_* */
     int32_t pad;
-    double p[4];      /* LJ: eps sigma R; ELEC: k R; GRAV: G; RF: k R eps_rf */
+    double p[4];      /* LJ: eps sigma R; ELEC: k R; GRAV: G; RF: k R eps_rf; MAG: mu/4pi */
 } orc_term;
 
 typedef struct {
@@ -52,6 +52,7 @@ typedef struct {
     int32_t pad2;
     double therm_T, therm_p;  /* Andersen: nu; Berendsen: tau; Langevin: gamma */
     double kb;
+    const double* mm;    /* n x 3 magnetic moments (MAG), or NULL */
 } orc_system;
 
 /* src/boundary_conditions.jl:111-172 */
@@ -118,6 +119,14 @@ double orc_potential_energy(const orc_system* s, const double* u);
 /* src/nbody_simulation_result.jl:664-709; frames: nframes x (3 x n) */
 void orc_rdf(const double* frames, int64_t nframes, int64_t n, const int64_t* indxs,
              int64_t nidx, double L, int64_t maxbin, double* rs, double* gr);
+
+/* src/basic_potentials.jl:333-365 (adds into dv[0..2]); mm: n x 3 moments.  The reference
+ * takes r_i - r_j without boundary conditions. */
+void orc_mag_accel(double* dv, const double* rs, int64_t i, int64_t n, const double* ms,
+                   const double* mm, double mu_4pi);
+/* magnetic dipole-dipole energy mu/4pi sum_{i<j} [m_i.m_j - 3 (m_i.r)(m_j.r)] / |r_ij|^3
+ * (r = unit r_ij; the reference has no such energy function) */
+double orc_mag_potential(const double* rs, int64_t n, const double* mm, double mu_4pi);
 
 /* Learned-stride descriptor (no reference counterpart): for every atom of a cubic box of
  * side L, 32 Gaussian radial bins (centres 0.8 + 1.7 b / 31, standard deviation 0.1,

@@ -15,7 +15,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
-TERM_LJ, TERM_ELEC, TERM_GRAV, TERM_RF = 1, 2, 3, 4
+TERM_LJ, TERM_ELEC, TERM_GRAV, TERM_RF, TERM_MAG = 1, 2, 3, 4, 5
 THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN = 0, 1, 2, 3
 
 
@@ -34,7 +34,7 @@ class OrcSystem(C.Structure):
                 ("water_elec", OrcTerm), ("rOH", C.c_double), ("aHOH", C.c_double),
                 ("kbond", C.c_double), ("kangle", C.c_double), ("therm", C.c_int32),
                 ("pad2", C.c_int32), ("therm_T", C.c_double), ("therm_p", C.c_double),
-                ("kb", C.c_double)]
+                ("kb", C.c_double), ("mm", C.POINTER(C.c_double))]
 
 
 _DP = C.POINTER(C.c_double)
@@ -73,6 +73,8 @@ def load():
                            _DP, _DP]),
         "orc_philox4x32": (None, [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]),
         "orc_lr_features": (None, [_DP, _DP, _DP, C.c_int64, C.c_double, _DP, _DP]),
+        "orc_mag_accel": (None, [_DP, _DP, C.c_int64, C.c_int64, _DP, _DP, C.c_double]),
+        "orc_mag_potential": (C.c_double, [_DP, C.c_int64, _DP, C.c_double]),
         "orc_u01": (C.c_double, [C.c_uint32, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
@@ -115,7 +117,15 @@ class System:
         s.qs = dp(self.qs) if self.qs is not None else C.POINTER(C.c_double)()
         s.bc = bc if bc is not None else make_bc()
         s.kb = kb
+        s.mm = C.POINTER(C.c_double)()
+        self.mm = None
         self.s = s
+
+    def moments(self, mm):
+        """Magnetic moments (n x 3) for MAG terms."""
+        self.mm = c3n(mm)
+        self.s.mm = dp(self.mm)
+        return self
 
     def add(self, kind, *p):
         t = self.s.terms[self.s.nterms]
@@ -204,3 +214,8 @@ def lr_features(u, a, v, L):
     fr = np.empty((n, 6))
     lib.orc_lr_features(dp(u), dp(a), dp(v), n, float(L), dp(f), dp(fr))
     return f, fr
+
+
+def mag_potential(u, mm, mu_4pi) -> float:
+    u, mm = c3n(u), c3n(mm)
+    return load().orc_mag_potential(dp(u), u.shape[0], dp(mm), float(mu_4pi))

@@ -388,3 +388,44 @@ def test_rdf_of_ideal_gas_is_one():
 ])
 def test_philox_known_answers(ctr, key, expect):
     assert tuple(O.philox(ctr, key)) == expect
+
+
+# ---------------------------------------------------------------------------------
+# Magnetic dipoles (src/basic_potentials.jl:333-365)
+# ---------------------------------------------------------------------------------
+
+def test_mag_two_dipoles_closed_form(oracle):
+    # parallel moments along z, separation along x: F_1 = -3 mu m^2 / d^4 x_hat
+    # (U = mu m^2 / d^3 falls with d, so the dipoles repel)
+    d, m, mu, mass = 0.7, 1.3, 0.9, 2.0
+    u = np.array([[-d / 2, 0.0, 0.0], [d / 2, 0.0, 0.0]])
+    mm = np.array([[0.0, 0.0, m], [0.0, 0.0, m]])
+    s = oracle.System([mass, mass]).moments(mm).add(oracle.TERM_MAG, mu)
+    a = s.rhs(u)
+    F = 3 * mu * m * m / d ** 4
+    assert np.allclose(a[0], [-F / mass, 0, 0], rtol=1e-14)
+    assert np.allclose(a[1], [F / mass, 0, 0], rtol=1e-14)
+    # head-to-tail along the separation: attraction of magnitude 6 mu m^2 / d^4
+    mm2 = np.array([[m, 0.0, 0.0], [m, 0.0, 0.0]])
+    a2 = oracle.System([mass, mass]).moments(mm2).add(oracle.TERM_MAG, mu).rhs(u)
+    assert np.allclose(a2[0], [6 * mu * m * m / d ** 4 / mass, 0, 0], rtol=1e-14)
+
+
+def test_mag_force_is_minus_energy_gradient(oracle):
+    rng = np.random.default_rng(31)
+    n = 6
+    u = rng.uniform(-1.5, 1.5, size=(n, 3))
+    mm = rng.normal(size=(n, 3))
+    ms = rng.uniform(0.5, 2.0, n)
+    mu = 0.37
+    a = oracle.System(ms).moments(mm).add(oracle.TERM_MAG, mu).rhs(u)
+    h = 1e-6
+    for i in range(n):
+        for k in range(3):
+            up, um = u.copy(), u.copy()
+            up[i, k] += h
+            um[i, k] -= h
+            g = (oracle.mag_potential(up, mm, mu) - oracle.mag_potential(um, mm, mu)) / (2 * h)
+            assert abs(-g / ms[i] - a[i, k]) <= 1e-6 * max(1.0, abs(a[i, k])), (i, k)
+    # Newton's third law: total force vanishes
+    assert np.abs((a * ms[:, None]).sum(0)).max() <= 1e-10 * np.abs(a * ms[:, None]).max()
