@@ -62,6 +62,7 @@ extern "C" {
 #define NBX_THERM_ANDERSEN 1  /* param = collision frequency nu */
 #define NBX_THERM_BERENDSEN 2 /* param = coupling time tau      */
 #define NBX_THERM_LANGEVIN 3  /* param = friction gamma         */
+#define NBX_THERM_NOSEHOOVER 4 /* param = relaxation time tau    */
 
 typedef struct nbx_ctx nbx_ctx;
 

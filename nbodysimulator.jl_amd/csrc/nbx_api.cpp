@@ -279,10 +279,20 @@ extern "C" int nbx_set_spcfw(nbx_ctx* c, double rOH, double aHOH, double kbond, 
     return NBX_OK;
 }
 
+// the extra coordinate column of the reference's Nose-Hoover state starts at zero
+// (gather_bodies_initial_coordinates, src/nbody_to_ode.jl:1-20)
+static int reset_nosehoover(nbx_ctx* c) {
+    NBX_HIP(c, hipMemsetAsync(c->d_scal + NBX_SCAL_ZETA, 0, 2 * sizeof(double), c->stream));
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
+    return NBX_OK;
+}
+
 extern "C" int nbx_set_thermostat(nbx_ctx* c, int32_t kind, double T, double param) {
     if (!c) return NBX_ERR_INVALID;
-    if (kind < NBX_THERM_NONE || kind > NBX_THERM_LANGEVIN)
+    if (kind < NBX_THERM_NONE || kind > NBX_THERM_NOSEHOOVER)
         return nbx_fail(c, NBX_ERR_INVALID, "unknown thermostat kind");
+    if (kind == NBX_THERM_NOSEHOOVER && T <= 0.0)
+        return nbx_fail(c, NBX_ERR_INVALID, "Nose-Hoover needs a target temperature > 0");
     if (kind != NBX_THERM_NONE) {
         if (!(T >= 0.0) || !std::isfinite(T))
             return nbx_fail(c, NBX_ERR_INVALID, "thermostat temperature must be >= 0");
@@ -291,7 +301,7 @@ extern "C" int nbx_set_thermostat(nbx_ctx* c, int32_t kind, double T, double par
     }
     c->therm = kind; c->therm_T = T; c->therm_p = param;
     invalidate(c);
-    return NBX_OK;
+    return reset_nosehoover(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,7 +350,7 @@ extern "C" int nbx_set_state(nbx_ctx* c, const double* u, const double* v) {
     c->accel_valid = false;
     c->nl.ready = false;
     c->steps_taken = 0;
-    return NBX_OK;
+    return reset_nosehoover(c);
 }
 
 extern "C" int nbx_get_state(const nbx_ctx* c, double* u, double* v) {
@@ -404,7 +414,8 @@ static void ref_term(nbx_ctx* c, const nbx_term& t, const double* x, double* dv,
 }
 
 // soode_system!(dv, v, u): dv overwritten (src/nbody_to_ode.jl:474-488, :502-532)
-static int ref_rhs(nbx_ctx* c, const double* x, const double* v, double* dv) {
+// xi_out: where the Nose-Hoover rate goes (the context's during runs, scratch otherwise)
+static int ref_rhs(nbx_ctx* c, const double* x, const double* v, double* dv, double* xi_out) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     ref64_fill(s, dv, 3 * n, 0.0);
@@ -416,16 +427,17 @@ static int ref_rhs(nbx_ctx* c, const double* x, const double* v, double* dv) {
         ref64_bond(s, x, dv, c->d_ms, n, 0, n, c->rOH, c->kbond);
         ref_term(c, c->water_lj, x, dv, 0, n);
         ref64_angle(s, x, dv, c->d_ms, n, 0, n / 3, c->aHOH, c->kangle);
-        if (c->therm == NBX_THERM_BERENDSEN) {
-            ref64_temperature(s, v, c->d_ms, n, c->kb, 2 * (n / 3), c->d_red, c->d_scal);
-            ref64_berendsen(s, dv, v, n, c->d_scal, c->therm_T, 0.5 / c->therm_p);
-        }
     } else {
         for (const nbx_term& t : c->terms) ref_term(c, t, x, dv, 0, n);
-        if (c->therm == NBX_THERM_BERENDSEN) {
-            ref64_temperature(s, v, c->d_ms, n, c->kb, 0, c->d_red, c->d_scal);
-            ref64_berendsen(s, dv, v, n, c->d_scal, c->therm_T, 0.5 / c->therm_p);
-        }
+    }
+    const int64_t Nc = c->sys == NBX_SYS_WATER ? 2 * (n / 3) : 0;
+    if (c->therm == NBX_THERM_BERENDSEN) {
+        ref64_temperature(s, v, c->d_ms, n, c->kb, Nc, c->d_red, c->d_scal);
+        ref64_berendsen(s, dv, v, n, c->d_scal, c->therm_T, 0.5 / c->therm_p);
+    } else if (c->therm == NBX_THERM_NOSEHOOVER) {
+        // src/thermostats.jl:116-122 (md_temperature over the n atoms, ndf = 3n - Nc)
+        ref64_temperature(s, v, c->d_ms, n, c->kb, Nc, c->d_red, c->d_scal);
+        ref64_nosehoover(s, dv, v, n, c->d_scal, xi_out, c->therm_T, c->therm_p, 3 * n - Nc);
     }
     NBX_HIP(c, hipGetLastError());
     return NBX_OK;
@@ -452,14 +464,14 @@ extern "C" int nbx_rhs(nbx_ctx* c, const double* u, const double* v, double* dv)
     int rc = check_config(c);
     if (rc) return rc;
     if (!all_finite(u, 3 * c->n)) return nbx_fail(c, NBX_ERR_NONFINITE, "u holds NaN or Inf");
-    if (c->therm == NBX_THERM_BERENDSEN && !v)
-        return nbx_fail(c, NBX_ERR_INVALID, "the Berendsen term needs v");
+    if ((c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER) && !v)
+        return nbx_fail(c, NBX_ERR_INVALID, "the Berendsen / Nose-Hoover term needs v");
     scratch3 s;
     if ((rc = alloc_scratch(c, s))) return rc;
     if ((rc = upload_aos(c, u, s.x))) return rc;
     if (v) { if ((rc = upload_aos(c, v, s.v))) return rc; }
     else ref64_fill(c->stream, s.v, 3 * c->n, 0.0);
-    if ((rc = ref_rhs(c, s.x, s.v, s.dv))) return rc;
+    if ((rc = ref_rhs(c, s.x, s.v, s.dv, c->d_scal + NBX_SCAL_XI_SCRATCH))) return rc;
     return download_aos(c, s.dv, dv);
 }
 
@@ -520,7 +532,7 @@ static int ref_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t 
     const int64_t n = c->n;
     int rc;
     if (!c->accel_valid) {
-        if ((rc = ref_rhs(c, c->d_x, c->d_v, c->d_a))) return rc;
+        if ((rc = ref_rhs(c, c->d_x, c->d_v, c->d_a, c->d_scal + NBX_SCAL_XI))) return rc;
         c->accel_valid = true;
     }
     const int water = c->sys == NBX_SYS_WATER;
@@ -530,7 +542,8 @@ static int ref_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t 
             ref64_langevin_half(s, c->d_v, c->d_ms, n, c->kb, c->therm_T, c->therm_p, dt, seed,
                                 step, NBX_RNG_LANGEVIN_A);
         ref64_drift(s, c->d_x, c->d_v, c->d_a, c->d_vh, n, dt);
-        if ((rc = ref_rhs(c, c->d_x, c->d_vh, c->d_a))) return rc;
+        if (c->therm == NBX_THERM_NOSEHOOVER) ref64_nh_drift(s, c->d_scal, dt);
+        if ((rc = ref_rhs(c, c->d_x, c->d_vh, c->d_a, c->d_scal + NBX_SCAL_XI))) return rc;
         ref64_kick(s, c->d_v, c->d_vh, c->d_a, n, dt);
         if (c->therm == NBX_THERM_LANGEVIN)
             ref64_langevin_half(s, c->d_v, c->d_ms, n, c->kb, c->therm_T, c->therm_p, dt, seed,

@@ -88,7 +88,7 @@ k_grav_finish(const double* __restrict__ part, int chunks, int n, double G, mx_k
             s[2] += pc[2 * n + i];
         }
         double ax = G * s[0], ay = G * s[1], az = G * s[2];
-        if (kp.therm == NBX_THERM_BERENDSEN) {
+        if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
             ax += cb * io.vh[i]; ay += cb * io.vh[n + i]; az += cb * io.vh[2 * n + i];
         }

@@ -10,6 +10,11 @@
 #include "nbx.h"
 #include "nbx_device.h"
 
+// device scalars (nbx_ctx::d_scal): [0] thermostat temperature, [1..7] energies, [8] finite
+// flag, [9] temperature, [10] Nose-Hoover zeta, [11] its rate xi, [12] pair count,
+// [13..14] list energies, [15] scratch rate for state-free RHS calls
+enum { NBX_SCAL_T = 0, NBX_SCAL_ZETA = 10, NBX_SCAL_XI = 11, NBX_SCAL_XI_SCRATCH = 15 };
+
 enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4, NBX_TERM_MAG = 5 };
 
 struct nbx_term {
@@ -136,6 +141,12 @@ void ref64_rf(hipStream_t s, const double* x, double* dv, const double* ms, cons
               double eps_rf, const nbx_bcv& bc);
 void ref64_mag(hipStream_t s, const double* x, double* dv, const double* ms, const double* mm,
                int64_t n, int64_t ib, int64_t ie, double mu_4pi);
+// Nose-Hoover (src/thermostats.jl:116-122): dv -= zeta v; xi_out = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
+// with T = scal[0] (computed by ref64_temperature) and zeta = scal[NBX_SCAL_ZETA]
+void ref64_nosehoover(hipStream_t s, double* dv, const double* v, int64_t n, const double* scal,
+                      double* xi_out, double T0, double tau, int64_t ndf);
+// zeta += dt xi (the extended coordinate's drift; its acceleration is zero)
+void ref64_nh_drift(hipStream_t s, double* scal, double dt);
 void ref64_grav(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,
                 int64_t ib, int64_t ie, double G);
 void ref64_bond(hipStream_t s, const double* x, double* dv, const double* ms, int64_t n,

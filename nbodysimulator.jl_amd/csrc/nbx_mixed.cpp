@@ -233,7 +233,8 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift) {
         io.req_out = req_flag(c, par ^ 1);
         io.L = c->bc.b[0];
     }
-    io.need_T = (c->therm == NBX_THERM_BERENDSEN && do_drift) ? 1 : 0;
+    io.need_T = ((c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER) && do_drift)
+                    ? 1 : 0;
     io.red_part = c->d_red;
     io.red_counter = reinterpret_cast<unsigned*>(c->nl.d_flags + 4);
     io.T_out = c->d_scal;
@@ -241,16 +242,28 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift) {
     return io;
 }
 
+mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
+    mx_nh_params nh{};
+    nh.nh = c->therm == NBX_THERM_NOSEHOOVER ? 1 : 0;
+    nh.advance = advance;
+    nh.dt = dt;
+    nh.T0 = c->therm_T;
+    nh.tau = c->therm_p;
+    nh.ndf = 3 * c->n - (c->sys == NBX_SYS_WATER ? 2 * (c->n / 3) : 0);
+    return nh;
+}
+
 // after a kernel that wrote x into X[par_out]: bins for a requested list build, and the
-// Berendsen temperature from the kernel's `nparts` per-block partials
-void after_drift(nbx_ctx* c, const mx_plan& p, int par_out, int nparts) {
+// thermostat temperature of vh from the kernel's `nparts` per-block partials (Berendsen;
+// Nose-Hoover also advances zeta by dt and sets its rate)
+void after_drift(nbx_ctx* c, const mx_plan& p, int par_out, int nparts, double dt) {
     hipStream_t s = c->stream;
     if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3)
         mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
                       bin_atoms(c, par_out));
-    if (c->therm == NBX_THERM_BERENDSEN)
+    if (c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER)
         mx_launch_T_final(s, c->d_red, nparts, c->kb * (double)(3 * c->n - constraints(c)),
-                          c->d_scal);
+                          c->d_scal, nh_params(c, dt, 1));
 }
 
 // blocks of the kernel that ends a step (its per-block temperature partials)
@@ -276,7 +289,7 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
         mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io, clear);
         if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
     }
-    if (do_drift) after_drift(c, p, par ^ 1, step_blocks(c, p));
+    if (do_drift) after_drift(c, p, par ^ 1, step_blocks(c, p), kp.dt);
 }
 
 void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
@@ -346,8 +359,9 @@ static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
 static int prologue(nbx_ctx* c, mx_plan& p) {
     hipStream_t s = c->stream;
     refresh_positions(c, p, true);
-    if (c->therm == NBX_THERM_BERENDSEN)
+    if (c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER)
         mx_launch_temperature(s, c, c->d_v, constraints(c), c->d_scal);
+    if (c->therm == NBX_THERM_NOSEHOOVER) mx_launch_nh_rate(s, c->d_scal, nh_params(c, 0.0, 0));
     // the Berendsen term of the first evaluation uses v itself (vh := v)
     NBX_HIP(c, hipMemcpyAsync(c->d_vh, c->d_v, sizeof(double) * 3 * (size_t)c->n,
                               hipMemcpyDeviceToDevice, s));
@@ -401,7 +415,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     backup4 bk;
-    NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * 4 * n3));
+    NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * (4 * n3 + 16)));
     const int64_t chunk_max = 500;
     // steps k = 1..count of the chunk, absolute step *d_step + k; then *d_step += count
     auto enqueue_steps = [&](int64_t k0, int64_t count) {
@@ -456,6 +470,8 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         for (int k = 0; k < 4; ++k)
             NBX_HIP(c, hipMemcpyAsync(bk.p + k * n3, src[k], sizeof(double) * n3,
                                       hipMemcpyDeviceToDevice, s));
+        NBX_HIP(c, hipMemcpyAsync(bk.p + 4 * n3, c->d_scal, sizeof(double) * 16,
+                                  hipMemcpyDeviceToDevice, s));
         const int64_t step0 = c->steps_taken;
         for (int attempt = 0;; ++attempt) {
             mx_launch_step_advance(s, c->d_step, step0, 1);
@@ -470,7 +486,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 kp.step_base = c->d_step;
                 kp.step_off = 0;
                 mx_launch_drift(s, c, kp, make_io(c, p, 0, 1));
-                after_drift(c, p, 1, (int)nbx_blocks(n, 256));
+                after_drift(c, p, 1, (int)nbx_blocks(n, 256), dt);
             }
             const int gs = c->graph_steps;
             int64_t k0 = 0;
@@ -492,6 +508,8 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
             for (int k = 0; k < 4; ++k)
                 NBX_HIP(c, hipMemcpyAsync(dst[k], bk.p + k * n3, sizeof(double) * n3,
                                           hipMemcpyDeviceToDevice, s));
+            NBX_HIP(c, hipMemcpyAsync(c->d_scal, bk.p + 4 * n3, sizeof(double) * 16,
+                                      hipMemcpyDeviceToDevice, s));
             if ((rc = grow_lists(c, p, grow))) return rc;
             if ((rc = prologue(c, p))) return rc;
             if (attempt > 8)
@@ -851,7 +869,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         ip.part = mpart.as<double>();
         lr_launch_mlp_infer(s, ip);
         lr_launch_drift(s, mpart.as<double>(), nblk, dlab.as<float>(), D, make_io(c, p, par, 1), n);
-        after_drift(c, p, par ^ 1, 0);
+        after_drift(c, p, par ^ 1, 0, D);
     };
     auto run_chunk = [&](int64_t msteps) -> int {
         int rc2;
@@ -966,10 +984,12 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     hipStream_t s = c->stream;
-    dev_buf bx, bv;
+    dev_buf bx, bv, bs;
     int rc;
     if ((rc = dalloc(c, bx, sizeof(double) * n3))) return rc;
     if ((rc = dalloc(c, bv, sizeof(double) * n3))) return rc;
+    if ((rc = dalloc(c, bs, sizeof(double) * 16))) return rc;
+    NBX_HIP(c, hipMemcpyAsync(bs.p, c->d_scal, sizeof(double) * 16, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipMemcpyAsync(bx.p, c->d_x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipMemcpyAsync(bv.p, c->d_v, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipStreamSynchronize(s));
@@ -977,6 +997,7 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     auto restore = [&]() -> int {
         NBX_HIP(c, hipMemcpyAsync(c->d_x, bx.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipMemcpyAsync(c->d_v, bv.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        NBX_HIP(c, hipMemcpyAsync(c->d_scal, bs.p, sizeof(double) * 16, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipStreamSynchronize(s));
         c->steps_taken = steps0;
         c->accel_valid = false;

@@ -151,8 +151,14 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
     return mi * (hx * hx + hy * hy + hz * hz);
 }
 
-// Berendsen coefficient gamma (T0/T - 1) (src/thermostats.jl:76-83)
+// Thermostat force a += c vh: Berendsen c = gamma (T0/T - 1) (src/thermostats.jl:76-83),
+// Nose-Hoover c = -zeta (src/thermostats.jl:116-122)
+__device__ __forceinline__ bool mx_therm_force(const mx_kick_params& kp) {
+    return kp.therm == NBX_THERM_BERENDSEN || kp.therm == NBX_THERM_NOSEHOOVER;
+}
+
 __device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
+    if (kp.therm == NBX_THERM_NOSEHOOVER) return -kp.d_T[NBX_SCAL_ZETA];
     const double T = kp.d_T[0];
     if (1.0 / T == __builtin_inf()) return kp.ber_gamma;
     return kp.ber_gamma * (kp.T / T - 1.0);
@@ -215,5 +221,14 @@ void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value);
 // bins of P (gated by *req; the bins were cleared by the preceding build kernel)
 void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* req,
                    int* bin_count, int* bin_atoms);
-// T = (sum of nparts per-block partials, in order) / denom
-void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* T);
+// T = (sum of nparts per-block partials, in order) / denom into scal[0]; with a
+// Nose-Hoover thermostat (nh != 0) also zeta += dt xi (advance != 0) and then
+// xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
+struct mx_nh_params {
+    int nh, advance;
+    double dt, T0, tau;
+    int64_t ndf;
+};
+void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* scal,
+                       const mx_nh_params& nh);
+void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh);

@@ -82,7 +82,7 @@ __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* 
         io.a[i] = ax; io.a[n + i] = ay; io.a[2 * n + i] = az;
         return 0.0;
     }
-    if (kp.therm == NBX_THERM_BERENDSEN) {
+    if (mx_therm_force(kp)) {
         const double cb = mx_berendsen_coeff(kp);
         ax += cb * io.vh[i]; ay += cb * io.vh[n + i]; az += cb * io.vh[2 * n + i];
     }
@@ -346,7 +346,8 @@ void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* 
 
 // temperature from the per-block partials of the preceding step kernel (block order)
 __global__ void __launch_bounds__(256)
-k_T_final(const double* __restrict__ part, int nparts, double denom, double* __restrict__ T) {
+k_T_final(const double* __restrict__ part, int nparts, double denom, double* __restrict__ scal,
+          mx_nh_params nh) {
     __shared__ double red[256];
     const int t = threadIdx.x;
     double s = 0.0;
@@ -357,11 +358,31 @@ k_T_final(const double* __restrict__ part, int nparts, double denom, double* __r
         if (t < off) red[t] += red[t + off];
         __syncthreads();
     }
-    if (t == 0) T[0] = red[0] / denom;
+    if (t == 0) {
+        const double T = red[0] / denom;
+        scal[0] = T;
+        if (nh.nh) {
+            if (nh.advance) scal[NBX_SCAL_ZETA] = scal[NBX_SCAL_ZETA] + nh.dt * scal[NBX_SCAL_XI];
+            const double it = 1.0 / nh.tau;
+            scal[NBX_SCAL_XI] = (it * it) * (T / nh.T0 - (double)(nh.ndf + 1) / (double)nh.ndf);
+        }
+    }
 }
 
-void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* T) {
-    hipLaunchKernelGGL(k_T_final, dim3(1), dim3(256), 0, s, part, nparts, denom, T);
+void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* scal,
+                       const mx_nh_params& nh) {
+    hipLaunchKernelGGL(k_T_final, dim3(1), dim3(256), 0, s, part, nparts, denom, scal, nh);
+}
+
+// xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf) from scal[0] (the initial right-hand side)
+__global__ void k_nh_rate(double* scal, mx_nh_params nh) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double it = 1.0 / nh.tau;
+    scal[NBX_SCAL_XI] = (it * it) * (scal[0] / nh.T0 - (double)(nh.ndf + 1) / (double)nh.ndf);
+}
+
+void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
+    hipLaunchKernelGGL(k_nh_rate, dim3(1), dim3(64), 0, s, scal, nh);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,7 +670,7 @@ k_water_molecules(int n, double rOH, double kbond, double aHOH, double ka, mx_ki
             acc[0][k] += -(fa + fc) / io.ms[o];
             acc[2][k] += fc / io.ms[o + 2];
         }
-        if (kp.therm == NBX_THERM_BERENDSEN) {
+        if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
             for (int t = 0; t < 3; ++t)
                 for (int k = 0; k < 3; ++k) acc[t][k] += cb * io.vh[k * n + o + t];

@@ -721,3 +721,37 @@ int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d
                        d_flag);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Nose-Hoover thermostat: src/thermostats.jl:116-122.  The reference keeps zeta in the
+// position part of an extra coordinate column and writes its rate into the velocity part
+// on every right-hand side; here both live in device scalars.
+// ---------------------------------------------------------------------------
+__global__ void k_ref_nosehoover(double* __restrict__ dv, const double* __restrict__ v, int64_t n,
+                                 const double* __restrict__ scal, double* __restrict__ xi_out,
+                                 double T0, double tau, int64_t ndf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double zeta = scal[NBX_SCAL_ZETA];
+    if (i < 3 * n) dv[i] -= zeta * v[i];               // @. dv -= u[zind] * v
+    if (i == 0) {
+        const double it = 1.0 / tau;                     // inv(p.tau)^2
+        const double T = scal[NBX_SCAL_T];
+        xi_out[0] = (it * it) * (T / T0 - (double)(ndf + 1) / (double)ndf);
+    }
+}
+
+void ref64_nosehoover(hipStream_t s, double* dv, const double* v, int64_t n, const double* scal,
+                      double* xi_out, double T0, double tau, int64_t ndf) {
+    const int64_t cnt = 3 * n > 0 ? 3 * n : 1;
+    hipLaunchKernelGGL(k_ref_nosehoover, dim3(nbx_blocks(cnt, 256)), dim3(256), 0, s, dv, v, n,
+                       scal, xi_out, T0, tau, ndf);
+}
+
+__global__ void k_ref_nh_drift(double* scal, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        scal[NBX_SCAL_ZETA] = scal[NBX_SCAL_ZETA] + dt * scal[NBX_SCAL_XI];
+}
+
+void ref64_nh_drift(hipStream_t s, double* scal, double dt) {
+    hipLaunchKernelGGL(k_ref_nh_drift, dim3(1), dim3(64), 0, s, scal, dt);
+}

@@ -25,7 +25,7 @@ NBX_ERR_NONFINITE = -6
 BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
 SYS_POTENTIAL, SYS_WATER = 0, 1
 PREC_FP64_REF, PREC_MIXED = 0, 1
-THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN = 0, 1, 2, 3
+THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN, THERM_NOSEHOOVER = 0, 1, 2, 3, 4
 
 _P = C.c_void_p
 _D = C.c_double

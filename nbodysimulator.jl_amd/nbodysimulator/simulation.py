@@ -126,7 +126,7 @@ def _set_thermostat(ctx: _lib.Context, th):
     elif isinstance(th, LangevinThermostat):
         ctx.set_thermostat(_lib.THERM_LANGEVIN, th.T, th.γ)
     elif isinstance(th, NoseHooverThermostat):
-        raise NotImplementedError("the Nose-Hoover thermostat is not part of the GPU path")
+        ctx.set_thermostat(_lib.THERM_NOSEHOOVER, th.T, th.τ)
     else:
         raise NotImplementedError(f"thermostat {type(th).__name__}")
 

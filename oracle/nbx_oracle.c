@@ -321,7 +321,27 @@ static void apply_term(const orc_system* s, const orc_term* t, double* dvi, cons
     }
 }
 
+/* Nose-Hoover (src/thermostats.jl:116-122): dv -= zeta v; xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
+ * with T = md_temperature of v; the reference keeps zeta at u[3n+1] and writes xi into
+ * v[3n+1] on every right-hand side. */
+static void nosehoover(double* dv, const double* v, const orc_system* s, int64_t N, int64_t Nc,
+                       double zeta, double* xi_out) {
+    for (int64_t k = 0; k < 3 * N; ++k) dv[k] -= zeta * v[k];
+    const double T = orc_md_temperature(v, s->ms, s->kb, N, Nc);
+    const int64_t ndf = 3 * N - Nc;
+    const double it = 1.0 / s->therm_p;
+    if (xi_out) *xi_out = (it * it) * (T / s->therm_T - (double)(ndf + 1) / (double)ndf);
+}
+
+static void rhs_impl(const orc_system* s, const double* u, const double* v, double* dv,
+                     double zeta, double* xi_out);
+
 void orc_rhs(const orc_system* s, const double* u, const double* v, double* dv) {
+    rhs_impl(s, u, v, dv, s->nh_zeta, NULL);
+}
+
+static void rhs_impl(const orc_system* s, const double* u, const double* v, double* dv,
+                     double zeta, double* xi_out) {
     const int64_t n = s->n;
     if (!s->water) {
         int64_t* all = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
@@ -334,6 +354,7 @@ void orc_rhs(const orc_system* s, const double* u, const double* v, double* dv) 
         free(all);
         if (s->therm == ORC_THERM_BERENDSEN)
             orc_berendsen(dv, v, s->ms, s->kb, n, 0, s->therm_T, 0.5 / s->therm_p);
+        if (s->therm == ORC_THERM_NOSEHOOVER) nosehoover(dv, v, s, n, 0, zeta, xi_out);
         return;
     }
     const int64_t nmol = n / 3;
@@ -365,6 +386,8 @@ void orc_rhs(const orc_system* s, const double* u, const double* v, double* dv) 
     free(oidx);
     if (s->therm == ORC_THERM_BERENDSEN)             /* :401-408: N = 3n, Nc = 2n */
         orc_berendsen(dv, v, s->ms, s->kb, n, 2 * nmol, s->therm_T, 0.5 / s->therm_p);
+    if (s->therm == ORC_THERM_NOSEHOOVER)           /* :425-433: N = 3n, Nc = 2n */
+        nosehoover(dv, v, s, n, 2 * nmol, zeta, xi_out);
 }
 
 /* ---------------------------------------------------------------------------
@@ -478,14 +501,18 @@ int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t
     if (save_every > 0 && u_frames) {
         memcpy(u_frames, u, bytes); memcpy(v_frames, v, bytes); nf = 1;
     }
-    orc_rhs(s, u, v, a);
+    /* Nose-Hoover extra coordinate: zeta (position part) and its rate xi (velocity part),
+       rewritten by every right-hand side; its acceleration is zero */
+    double zeta = s->nh_zeta, xi = 0.0;
+    rhs_impl(s, u, v, a, zeta, &xi);
     for (int64_t step = 1; step <= nsteps; ++step) {
         if (s->therm == ORC_THERM_LANGEVIN) langevin_half(s, v, dt, step, seed, RNG_LANGEVIN_A);
         for (int64_t k = 0; k < n3; ++k) {
             u[k] = u[k] + dt * v[k] + dtsq * (half * a[k]);
             vh[k] = v[k] + dt * (half * a[k]);
         }
-        orc_rhs(s, u, vh, an);
+        zeta = zeta + dt * xi;
+        rhs_impl(s, u, vh, an, zeta, &xi);
         for (int64_t k = 0; k < n3; ++k) v[k] = vh[k] + dt * (half * an[k]);
         double* t = a; a = an; an = t;
         if (s->therm == ORC_THERM_LANGEVIN) langevin_half(s, v, dt, step, seed, RNG_LANGEVIN_B);

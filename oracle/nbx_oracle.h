@@ -22,7 +22,7 @@ extern "C" {
 enum { ORC_BC_INFINITE = 0, ORC_BC_CUBIC = 1, ORC_BC_PERIODIC = 2 };
 enum { ORC_TERM_LJ = 1, ORC_TERM_ELEC = 2, ORC_TERM_GRAV = 3, ORC_TERM_RF = 4, ORC_TERM_MAG = 5 };
 enum { ORC_THERM_NONE = 0, ORC_THERM_ANDERSEN = 1, ORC_THERM_BERENDSEN = 2,
-       ORC_THERM_LANGEVIN = 3 };
+       ORC_THERM_LANGEVIN = 3, ORC_THERM_NOSEHOOVER = 4 };
 
 typedef struct {
     int32_t kind;     /* ORC_BC_* */
@@ -50,9 +50,10 @@ typedef struct {
     /* thermostat */
     int32_t therm;       /* ORC_THERM_* */
     int32_t pad2;
-    double therm_T, therm_p;  /* Andersen: nu; Berendsen: tau; Langevin: gamma */
+    double therm_T, therm_p;  /* Andersen: nu; Berendsen, Nose-Hoover: tau; Langevin: gamma */
     double kb;
     const double* mm;    /* n x 3 magnetic moments (MAG), or NULL */
+    double nh_zeta;      /* Nose-Hoover zeta at the start (the reference's u[3n+1] = 0) */
 } orc_system;
 
 /* src/boundary_conditions.jl:111-172 */

@@ -16,7 +16,7 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
 TERM_LJ, TERM_ELEC, TERM_GRAV, TERM_RF, TERM_MAG = 1, 2, 3, 4, 5
-THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN = 0, 1, 2, 3
+THERM_NONE, THERM_ANDERSEN, THERM_BERENDSEN, THERM_LANGEVIN, THERM_NOSEHOOVER = 0, 1, 2, 3, 4
 
 
 class OrcBC(C.Structure):
@@ -34,7 +34,7 @@ class OrcSystem(C.Structure):
                 ("water_elec", OrcTerm), ("rOH", C.c_double), ("aHOH", C.c_double),
                 ("kbond", C.c_double), ("kangle", C.c_double), ("therm", C.c_int32),
                 ("pad2", C.c_int32), ("therm_T", C.c_double), ("therm_p", C.c_double),
-                ("kb", C.c_double), ("mm", C.POINTER(C.c_double))]
+                ("kb", C.c_double), ("mm", C.POINTER(C.c_double)), ("nh_zeta", C.c_double)]
 
 
 _DP = C.POINTER(C.c_double)

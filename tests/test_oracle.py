@@ -429,3 +429,43 @@ def test_mag_force_is_minus_energy_gradient(oracle):
             assert abs(-g / ms[i] - a[i, k]) <= 1e-6 * max(1.0, abs(a[i, k])), (i, k)
     # Newton's third law: total force vanishes
     assert np.abs((a * ms[:, None]).sum(0)).max() <= 1e-10 * np.abs(a * ms[:, None]).max()
+
+
+# ---------------------------------------------------------------------------------
+# Nose-Hoover (src/thermostats.jl:116-122)
+# ---------------------------------------------------------------------------------
+
+def test_nosehoover_free_particles_follow_the_thermostat_ode(oracle):
+    """Free particles under the reference's Nose-Hoover law: dT/dt = -2 zeta T and
+    dzeta/dt = (T/T0 - (ndf+1)/ndf) / tau^2 (zeta(0) = 0).  The oracle's velocity
+    Verlet at small dt follows a high-accuracy solution of that ODE; a wrong sign, a
+    missing 1/tau^2 or the plain T/T0 - 1 would not."""
+    from scipy.integrate import solve_ivp
+    n, m, kb, T0, tau = 50, 2.0, 1.0, 1.0, 0.5
+    rng = np.random.default_rng(41)
+    v = rng.normal(size=(n, 3))
+    u = rng.uniform(-5, 5, size=(n, 3))
+    ndf = 3 * n
+    Ti = m * (v * v).sum() / (kb * ndf)
+    s = oracle.System(np.full(n, m), kb=kb).thermostat(oracle.THERM_NOSEHOOVER, T0, tau)
+    r = (ndf + 1) / ndf
+    tend = 4.0
+    sol = solve_ivp(lambda t, y: [-2 * y[1], (np.exp(y[0]) / T0 - r) / tau ** 2],
+                    (0, tend), [np.log(Ti), 0.0], t_eval=np.linspace(0, tend, 11),
+                    rtol=1e-11, atol=1e-12)
+    Te = np.exp(sol.y[0])
+    errs = []
+    for dt in (2e-3, 1e-3, 5e-4):
+        steps = int(round(tend / dt))
+        _, _, (uf, vf) = s.vv_run(u, v, dt, steps, save_every=steps // 10)
+        Tg = np.array([m * (w * w).sum() / (kb * ndf) for w in vf])
+        errs.append(np.abs(Tg / Te - 1).max())
+    # the reference's coupling (zeta advanced with the previous rate) is first order in dt:
+    # the error halves with dt and goes to zero, i.e. the scheme converges to this ODE
+    assert errs[2] < 6e-3, errs
+    assert 1.6 < errs[0] / errs[1] < 2.5 and 1.6 < errs[1] / errs[2] < 2.5, errs
+    dt = 1e-3
+    # equilibrium point: T = T0 (ndf+1)/ndf keeps zeta at zero and the velocities fixed
+    v_eq = v * np.sqrt(T0 * r / Ti)
+    _, v1, _ = s.vv_run(u, v_eq, dt, 50)
+    assert np.allclose(v1, v_eq, rtol=1e-12, atol=1e-14)
