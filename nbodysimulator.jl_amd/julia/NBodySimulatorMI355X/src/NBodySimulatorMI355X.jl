@@ -25,7 +25,8 @@ using NBodySimulator: NBodySimulation, PotentialNBodySystem, WaterSPCFw, Lennard
                       ElectrostaticParameters, GravitationalParameters, SPCFwParameters,
                       CubicPeriodicBoundaryConditions, PeriodicBoundaryConditions, InfiniteBox,
                       AndersenThermostat, BerendsenThermostat, LangevinThermostat,
-                      NullThermostat, SimulationResult, get_masses
+                      NoseHooverThermostat, NullThermostat, SimulationResult, get_masses,
+                      MagnetostaticParameters, MagneticParticle
 import NBodySimulator: get_accelerating_function, run_simulation
 import SciMLBase
 using RecursiveArrayTools: ArrayPartition
@@ -37,8 +38,8 @@ const libnbx = get(ENV, "NBX_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "
 const NBX_BC_INFINITE, NBX_BC_CUBIC, NBX_BC_PERIODIC = Int32(0), Int32(1), Int32(2)
 const NBX_SYS_POTENTIAL, NBX_SYS_WATER = Int32(0), Int32(1)
 const NBX_PREC_FP64_REF, NBX_PREC_MIXED = Int32(0), Int32(1)
-const NBX_THERM_NONE, NBX_THERM_ANDERSEN, NBX_THERM_BERENDSEN, NBX_THERM_LANGEVIN =
-    Int32(0), Int32(1), Int32(2), Int32(3)
+const NBX_THERM_NONE, NBX_THERM_ANDERSEN, NBX_THERM_BERENDSEN, NBX_THERM_LANGEVIN,
+      NBX_THERM_NOSEHOOVER = Int32(0), Int32(1), Int32(2), Int32(3), Int32(4)
 
 """
     MI355XVerlet(; precision = :fp64, seed = 0, skin = nothing)
@@ -98,12 +99,23 @@ add_potential!(ctx, p::ElectrostaticParameters) =
                      ctx.ptr, p.k, p.R))
 add_potential!(ctx, p::GravitationalParameters) =
     check(ctx, ccall((:nbx_add_gravitational, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, p.G))
+add_potential!(ctx, p::MagnetostaticParameters) =
+    check(ctx, ccall((:nbx_add_magnetostatic, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, p.μ_4π))
 add_potential!(ctx, p) = throw(ArgumentError("no GPU kernel for $(typeof(p))"))
+
+function set_moments!(ctx, bodies)
+    mm = Matrix{Float64}(undef, 3, length(bodies))
+    for (i, b) in enumerate(bodies)
+        mm[:, i] .= b.mm
+    end
+    check(ctx, ccall((:nbx_set_moments, libnbx), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, mm))
+end
 
 function set_thermostat!(ctx, th)
     kind, T, par = th isa AndersenThermostat ? (NBX_THERM_ANDERSEN, th.T, th.ν) :
                    th isa BerendsenThermostat ? (NBX_THERM_BERENDSEN, th.T, th.τ) :
                    th isa LangevinThermostat ? (NBX_THERM_LANGEVIN, th.T, th.γ) :
+                   th isa NoseHooverThermostat ? (NBX_THERM_NOSEHOOVER, th.T, th.τ) :
                    th isa NullThermostat ? (NBX_THERM_NONE, 0.0, 1.0) :
                    throw(ArgumentError("thermostat $(typeof(th)) has no GPU kernel"))
     check(ctx, ccall((:nbx_set_thermostat, libnbx), Cint, (Ptr{Cvoid}, Int32, Cdouble, Cdouble),
@@ -134,6 +146,7 @@ function build_context(sim::NBodySimulation, precision::Symbol; only = nothing)
     else
         ctx = Context(length(ms), NBX_SYS_POTENTIAL, ms, charges(sys))
         set_boundary!(ctx, sim.boundary_conditions)
+        all(b -> b isa MagneticParticle, sys.bodies) && set_moments!(ctx, sys.bodies)
         for p in (only === nothing ? values(sys.potentials) : (only,))
             add_potential!(ctx, p)
         end
