@@ -169,8 +169,9 @@ function run_simulation(sim::NBodySimulation, alg::MI355XVerlet; dt::Real,
     alg.skin === nothing ||
         check(ctx, ccall((:nbx_set_skin, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, alg.skin))
     u0, v0, _ = NBodySimulator.gather_bodies_initial_coordinates(sim)
-    u0 = Matrix{Float64}(u0); v0 = Matrix{Float64}(v0)
     n = ctx.n
+    # a Nose-Hoover state carries an extra column (zeta); the library keeps zeta itself
+    u0 = Matrix{Float64}(u0[:, 1:n]); v0 = Matrix{Float64}(v0[:, 1:n])
     check(ctx, ccall((:nbx_set_state, libnbx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
                      ctx.ptr, u0, v0))
     t1, t2 = sim.tspan
