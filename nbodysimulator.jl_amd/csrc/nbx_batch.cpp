@@ -99,6 +99,10 @@ static void batch_release(nbx_batch* b) {
 static double rlist_of(const nbx_batch* b) { return b->R + b->skin; }
 
 static int alloc_lists(nbx_batch* b, double grow) {
+    // a captured step graph holds the list pointer: it must not outlive the list
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    ++b->version;
     bfree(b->d_nbr);
     b->d_nbr = nullptr;
     double rho = 0.0;

@@ -314,6 +314,13 @@ int mx_supported(nbx_ctx* c, std::string* why) {
 static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
     std::string why;
     if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+    // the lists (and gravity partials) below may be freed and allocated again; a captured
+    // step graph holds their addresses, so it must not be replayed after this point (its
+    // key cannot tell a reallocation that happens to return the same list address)
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey = nbx_graph_key();
+    ++c->version;
     const int64_t n = c->n;
     const size_t pb = sizeof(double) * 4 * (size_t)std::max<int64_t>(n, 1);
     if (!c->d_pos) NBX_HIP(c, hipMalloc(&c->d_pos, pb));
