@@ -168,6 +168,10 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
         ctx.set_kb(1.0)
         ctx.set_precision(mode[1])
         ctx.set_state(s.u, s.v)
+        tuned = None
+        if mode[0] == "mixed":
+            # launch shape (lanes per atom, list skin) picked by the autotuner
+            tuned = ctx.autotune(0.005, 200)
         ke0, pe0, _ = ctx.energies()
         nst = steps if mode[0] == "mixed" else max(steps // 10, 10)
         ctx.vv_run(0.005, 20)
@@ -178,6 +182,9 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
         pairs = ctx.count_pairs() if mode[0] == "mixed" else None
         rec = {"steps": nst, "steps_per_s": nst / el, "ms_per_step": 1e3 * el / nst,
                "energy_drift_rel": abs((ke1 + pe1) - (ke0 + pe0)) / abs(ke0 + pe0)}
+        if tuned:
+            rec["autotuned"] = {"lanes": int(tuned["shape0"]), "skin": tuned["shape1"],
+                                "ms_per_step_default": tuned["entry_ms_per_step"]}
         if pairs:
             rec["pairs_per_step"] = pairs
             rec["pair_interactions_per_s"] = pairs * nst / el
@@ -241,6 +248,7 @@ def bench_water(args, lib, gen):
     Rc = 0.9
     dt, steps = 0.0005, 2000
     ctx = water_context(lib, gen, w, Rc, True)
+    tuned = ctx.autotune(dt, 100)
     ctx.vv_run(dt, 20)
     t0 = time.perf_counter()
     ctx.vv_run(dt, steps)
@@ -264,6 +272,8 @@ def bench_water(args, lib, gen):
             "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
             "pairs_per_step": pairs, "pair_interactions_per_s": pairs * steps / el,
             "temperature_after_K": T1,
+            "autotuned": {"lanes": int(tuned["shape0"]), "skin_nm": tuned["shape1"],
+                          "ms_per_step_default": tuned["entry_ms_per_step"]},
             "force_kernel": {"name": kname, "avg_ms": kms},
             "nve_energy_drift_rel_10k_steps": abs((k1 + p1) - (k0 + p0)) / abs(k0 + p0),
             "force_error_vs_fp64_oracle": float(np.abs(a - ref).max() / np.abs(ref).max()),
@@ -381,6 +391,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
 
     # plain velocity Verlet at dt: the reference point and the judge
     plain, s = ctx_for(4242)
+    plain.autotune(dt, 200)
     ke0, pe0 = plain.learn_energy()
     plain.vv_run(dt, 20)
     plain.set_state(s.u, s.v)
