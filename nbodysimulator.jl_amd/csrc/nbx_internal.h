@@ -42,20 +42,30 @@ struct nbx_nlist {
     size_t cap_bytes = 0;
 };
 
-// What a captured step graph depends on; a mismatch forces a new capture.
+// What a captured step graph depends on; a mismatch forces a new capture.  Every device
+// buffer the captured kernels reference is part of the key (lists are freed and
+// allocated again when they grow or the configuration changes, and an allocation may or
+// may not return a previous address).
 struct nbx_graph_key {
     int valid = 0;
     int64_t version = 0;      // configuration version (bumped by every setter)
     double dt = 0.0;
     uint64_t seed = 0;
     const void* nbr = nullptr;
+    const void* nnb = nullptr;
+    const void* cell_count = nullptr;
+    const void* cell_atoms = nullptr;
+    const void* xref = nullptr;
+    const void* flags = nullptr;
     const void* pos = nullptr;
     const void* part = nullptr;
-    int cap = 0, ccap = 0, ncell = 0, lanes = 0, steps = 0;
+    int cap = 0, ccap = 0, ncell = 0, lanes = 0, steps = 0, chunks = 0, tpt = 0;
     bool operator==(const nbx_graph_key& o) const {
         return valid && o.valid && version == o.version && dt == o.dt && seed == o.seed &&
-               nbr == o.nbr && pos == o.pos && part == o.part && cap == o.cap &&
-               ccap == o.ccap && ncell == o.ncell && lanes == o.lanes && steps == o.steps;
+               nbr == o.nbr && nnb == o.nnb && cell_count == o.cell_count &&
+               cell_atoms == o.cell_atoms && xref == o.xref && flags == o.flags && pos == o.pos &&
+               part == o.part && cap == o.cap && ccap == o.ccap && ncell == o.ncell &&
+               lanes == o.lanes && steps == o.steps && chunks == o.chunks && tpt == o.tpt;
     }
 };
 

@@ -314,13 +314,6 @@ int mx_supported(nbx_ctx* c, std::string* why) {
 static int prepare(nbx_ctx* c, mx_plan& p, double grow) {
     std::string why;
     if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
-    // the lists (and gravity partials) below may be freed and allocated again; a captured
-    // step graph holds their addresses, so it must not be replayed after this point (its
-    // key cannot tell a reallocation that happens to return the same list address)
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-    c->gkey = nbx_graph_key();
-    ++c->version;
     const int64_t n = c->n;
     const size_t pb = sizeof(double) * 4 * (size_t)std::max<int64_t>(n, 1);
     if (!c->d_pos) NBX_HIP(c, hipMalloc(&c->d_pos, pb));
@@ -443,8 +436,15 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         key.dt = dt;
         key.seed = seed;
         key.nbr = c->nl.d_nbr;
+        key.nnb = c->nl.d_nnb;
+        key.cell_count = c->nl.d_cell_count;
+        key.cell_atoms = c->nl.d_cell_atoms;
+        key.xref = c->nl.d_xref;
+        key.flags = c->nl.d_flags;
         key.pos = c->d_pos;
         key.part = c->d_part;
+        key.chunks = c->grav_chunks;
+        key.tpt = c->grav_tpt_set;
         key.cap = c->nl.cap;
         key.ccap = c->nl.ccap;
         key.ncell = c->nl.ncell;
