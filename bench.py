@@ -125,7 +125,7 @@ def bench_gravity(args, lib, gen):
     kms, kname = ctx.time_force_kernel(20)
     # the same run continued to 10k steps for the long-run energy drift
     drift10k = None
-    if not args.no_secondary:
+    if not args.no_secondary and not args.no_long_drift:
         ctx.vv_run(dt, 10000 - args.steps - args.warmup)
         ke2, _, pg2 = ctx.energies()
         drift10k = abs((ke2 + pg2) - e0) / abs(e0)
@@ -467,6 +467,8 @@ def main():
     ap.add_argument("--profile-learned", action="store_true",
                     help="only run 200 strided steps (stride 2, zero network) of LJ N=32000 "
                          "for a kernel trace of the strided step")
+    ap.add_argument("--no-long-drift", action="store_true",
+                    help="skip the 10k-step gravity drift continuation (kernel-trace runs)")
     ap.add_argument("--learn-traj", type=int, default=200,
                     help="training trajectories of the learned-stride line")
     ap.add_argument("--profile-kernels", action="store_true",
