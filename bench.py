@@ -111,8 +111,9 @@ def bench_gravity(args, lib, gen):
     a_gpu = mixed_accel(ctx, u)[targets]
     a_ref, cpu_s = cpu_gravity_sample(u, m, targets)
     force_err = float(np.abs(a_gpu - a_ref).max() / np.abs(a_ref).max())
-    # timed run
+    # timed run, with the tile shape (targets per thread, source chunks) from the autotuner
     ctx.set_state(u, v)
+    tuned = ctx.autotune(dt, 20)
     ke0, _, pg0 = ctx.energies()
     ctx.vv_run(dt, args.warmup, seed=1)
     t0 = time.perf_counter()
@@ -142,7 +143,10 @@ def bench_gravity(args, lib, gen):
                    "force_error_vs_fp64_oracle": force_err,
                    "energy_drift_rel": abs(e1 - e0) / abs(e0),
                    "energy_drift_steps": args.warmup + args.steps,
-                   "energy_drift_rel_10k": drift10k},
+                   "energy_drift_rel_10k": drift10k,
+                   "autotuned": {"targets_per_thread": int(tuned["shape0"]),
+                                 "source_chunks": int(tuned["shape1"]),
+                                 "ms_per_step_default": tuned["entry_ms_per_step"]}},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic_from_profiles(kname),
@@ -429,10 +433,12 @@ def profile_kernels(lib, gen):
     ctx.add_gravitational(1.0)
     ctx.set_precision(lib.PREC_MIXED)
     ctx.set_state(u, v)
+    tuned = ctx.autotune(1e-3, 20)          # the same tile shape as the headline run
     kms, kname = ctx.time_force_kernel(10)
     nbytes = 1 << 30
     cms = ctx.copy_probe(nbytes, 5)
     print(json.dumps({"kernel": kname, "kernel_avg_ms": kms, "copy_bytes": nbytes,
+                      "targets_per_thread": int(tuned["shape0"]), "source_chunks": int(tuned["shape1"]),
                       "copy_avg_ms": cms, "copy_GBps": 2 * nbytes / (cms * 1e-3) / 1e9}),
           flush=True)
     return 0

@@ -9,8 +9,9 @@
 // accumulator, and chunk partials are stored in FP64 and added by the kick kernel in
 // chunk order (deterministic).
 //   a_i = G * sum_j m_j (r_j - r_i) / |r_j - r_i|^3      (= -G m_j r_ij / |r_ij|^3)
-// The self term is removed by clamping r^2 to 1e-20 before the reciprocal square root:
-// for j == i the displacement is exactly zero, so its contribution is exactly zero.
+// The self term is removed by adding 1e-20 to r^2 inside the FMA chain (no extra
+// instruction): for j == i the displacement is exactly zero, so its contribution is
+// exactly zero; for distinct bodies the addition rounds away.
 #include "nbx_mixed.h"
 
 #define GR_THREADS 256
@@ -47,8 +48,10 @@ k_grav_tiled(const float4* __restrict__ pos, int n, int jchunk, double* __restri
 #pragma unroll
             for (int t = 0; t < TPT; ++t) {
                 const float dx = pj.x - xi[t], dy = pj.y - yi[t], dz = pj.z - zi[t];
-                float r2 = dx * dx + dy * dy + dz * dz;
-                r2 = fmaxf(r2, 1e-20f);
+                // r^2 + 1e-20 inside the FMA chain: the self term (dx = dy = dz = 0) gets a
+                // finite 1/r^3 times a zero displacement; for any two distinct FP32
+                // positions r^2 >> 2^24 * 1e-20, so the sum rounds to r^2 exactly
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
                 const float rinv = __builtin_amdgcn_rsqf(r2);
                 const float s = pj.w * (rinv * rinv * rinv);
                 sx[t] += dx * s;
@@ -117,7 +120,10 @@ void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double 
     const int chunks = c->grav_chunks;
     const int jchunk = (int)((n + chunks - 1) / chunks);
     const dim3 grid(nbx_blocks(n, GR_THREADS * tpt), chunks);
-    if (tpt == 2)
+    if (tpt == 4)
+        hipLaunchKernelGGL(k_grav_tiled<4>, grid, dim3(GR_THREADS), 0, s, (const float4*)pos, n,
+                           jchunk, c->d_part);
+    else if (tpt == 2)
         hipLaunchKernelGGL(k_grav_tiled<2>, grid, dim3(GR_THREADS), 0, s, (const float4*)pos, n,
                            jchunk, c->d_part);
     else

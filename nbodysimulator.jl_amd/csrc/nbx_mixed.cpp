@@ -975,7 +975,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
 // a warm-up run long enough to capture its step graph; the state, the step count and the
 // integrator's inputs are restored after every trial, and the fastest candidate is kept.
 //   list plans   : lanes per atom {2, 4, 8, 16} x skin {0.06 .. 0.40} x cutoff
-//   gravity      : targets per thread {1, 2} x source chunks {1, 2, 4, ..., 64}
+//   gravity      : targets per thread {1, 2, 4} x source chunks {1, 2, 4, ..., 64}
 // out: [0] lanes | targets per thread, [1] skin | chunks, [2] best ms per step,
 //      [3] ms per step of the configuration found on entry, [4] candidates timed.
 // =================================================================================
@@ -1040,7 +1040,7 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         best_ms = base_ms;
         best0 = tpt0;
         best1 = ch0;
-        for (int tpt = 1; tpt <= 2; ++tpt)
+        for (int tpt = 1; tpt <= 4; tpt *= 2)
             for (int ch = 1; ch <= 64; ch *= 2) {
                 if (n / ch < 256) break;
                 c->grav_tpt_set = tpt;
