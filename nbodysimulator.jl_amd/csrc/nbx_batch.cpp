@@ -23,8 +23,8 @@ void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t
 void batch_fill_params(void* dst, double eps, double sigma, double R, double rlist, double skin,
                        double dt, double kb, int cap, int64_t ntot);
 void batch_launch_steps_raw(hipStream_t s, int nblocks, const void* reps, const int* order,
-                            double* x, double* v, double* a, double* vh, double* xref, int* nbr,
-                            int* nnb, const double* ms, int* scratch3, const void* params,
+                            double* x, double* v, double* a, double* vh, double* xref, uint16_t* nbr,
+                            int* nnb, const double* ms, int* slot_of, const void* params,
                             const int64_t* step_base, int nsteps, int prologue, int* flags);
 void batch_launch_energies(hipStream_t s, int nrep, const void* reps, const double* x,
                            const double* v, const double* ms, int64_t nt, double s2, double R2,
@@ -50,9 +50,9 @@ struct nbx_batch {
     double* d_xref = nullptr;
     double* d_ms = nullptr;
     double* d_tmp = nullptr;
-    int* d_nbr = nullptr;
+    uint16_t* d_nbr = nullptr;  // 16-bit slots in groups of 8 per atom
     int* d_nnb = nullptr;
-    int* d_scratch3 = nullptr; // slot of each atom, its cell, cell-ordered atom indices
+    int* d_slot = nullptr;    // LDS slot of each atom (cell order) since the last list build
     int* d_flags = nullptr;   // [0] overflow, [1] list builds
     int* d_order = nullptr;
     int64_t* d_step = nullptr;
@@ -88,7 +88,7 @@ static void batch_release(nbx_batch* b) {
     b->gexec = nullptr;
     bfree(b->d_x); bfree(b->d_v); bfree(b->d_a); bfree(b->d_vh); bfree(b->d_xref);
     bfree(b->d_ms); bfree(b->d_tmp); bfree(b->d_nbr); bfree(b->d_nnb); bfree(b->d_flags);
-    bfree(b->d_scratch3);
+    bfree(b->d_slot);
     bfree(b->d_order); bfree(b->d_step); bfree(b->d_reps);
     if (b->h_params) free(b->h_params);
     b->h_params = nullptr;
@@ -113,14 +113,17 @@ static int alloc_lists(nbx_batch* b, double grow) {
     }
     const double rl = rlist_of(b);
     const double est = 4.0 / 3.0 * M_PI * rl * rl * rl * rho;
-    b->cap = (int)std::min<double>((double)nmax, std::ceil(grow * (1.6 * est + 24.0)));
+    // a multiple of 8 entries (whole 16-byte groups), so every replica's list starts on
+    // a 16-byte boundary
+    const int cap = (int)std::min<double>((double)nmax, std::ceil(grow * (1.6 * est + 24.0)));
+    b->cap = (cap + 7) / 8 * 8;
     b->list_off.assign(b->nrep, 0);
     int64_t total = 0;
     for (int r = 0; r < b->nrep; ++r) {
         b->list_off[r] = total;
         total += (int64_t)b->cap * b->n[r];
     }
-    BHIP(b, hipMalloc((void**)&b->d_nbr, sizeof(int) * (size_t)std::max<int64_t>(total, 1)));
+    BHIP(b, hipMalloc((void**)&b->d_nbr, sizeof(uint16_t) * (size_t)std::max<int64_t>(total, 8)));
     return NBX_OK;
 }
 
@@ -177,7 +180,7 @@ extern "C" int nbx_batch_create(nbx_batch** out, int32_t nrep, const int64_t* na
         if (e == hipSuccess) e = hipMalloc((void**)p, b3);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_ms, sizeof(double) * (size_t)ntot);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_nnb, sizeof(int) * (size_t)ntot);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_scratch3, sizeof(int) * 3 * (size_t)ntot);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_slot, sizeof(int) * (size_t)ntot);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_flags, sizeof(int) * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_order, sizeof(int) * (size_t)nrep);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_step, sizeof(int64_t));
@@ -300,7 +303,7 @@ static int upload_tables(nbx_batch* b, double dt, uint64_t seed) {
 
 static void launch(nbx_batch* b, int nsteps, int prologue) {
     batch_launch_steps_raw(b->stream, b->nrep, b->d_reps, b->d_order, b->d_x, b->d_v, b->d_a,
-                           b->d_vh, b->d_xref, b->d_nbr, b->d_nnb, b->d_ms, b->d_scratch3,
+                           b->d_vh, b->d_xref, b->d_nbr, b->d_nnb, b->d_ms, b->d_slot,
                            b->h_params, b->d_step, nsteps, prologue, b->d_flags);
 }
 

@@ -6,8 +6,11 @@
 //   - x, v, a, vh and the list reference positions stay in FP64 in HBM (SoA over the
 //     whole batch: component k of global atom g at k * ntot + g); a thread owns atoms
 //     tid, tid + BATCH_THREADS, ... in every phase, so it reads back its own writes;
-//   - LDS positions are kept in cell order ("slots"); the Verlet list (entry k of local
-//     atom i at nbr[list_off + k * n + i]) holds slots.  It is rebuilt inside the launch
+//   - LDS positions are kept in cell order ("slots"); the Verlet list holds 16-bit slots
+//     in groups of 8 (entry k of local atom i at nbr[list_off + ((k / 8) * n + i) * 8 +
+//     k % 8]): a thread fetches 8 neighbours with one 16-byte load and prefetches the next
+//     group (and the next atom's first group) while it computes, so the pair loop is not
+//     a chain of dependent HBM round trips.  The list is rebuilt inside the launch
 //     whenever an atom of the replica moved more than skin/2 since the last build: atoms
 //     are binned into an LDS cell list (cells >= list radius), placed, rank-sorted by
 //     index within each cell (reproducible lists), and every atom scans the 27
@@ -21,6 +24,7 @@
 #define BATCH_THREADS 512
 #define BATCH_NMAX 4096
 #define BATCH_MAXCELLS 512
+#define BATCH_PER_THREAD (BATCH_NMAX / BATCH_THREADS)
 
 struct nbx_batch_rep {
     int64_t off;       // first global atom
@@ -54,33 +58,54 @@ __device__ __forceinline__ float4 batch_wrap(double x, double y, double z, doubl
     return make_float4((float)wx, (float)wy, (float)wz, 0.0f);
 }
 
-__global__ void __launch_bounds__(BATCH_THREADS)
+// cell of a wrapped position (nc cells per side, inv_cs = nc / L)
+__device__ __forceinline__ int batch_cell(float4 w, double inv_cs, int nc) {
+    int cx = (int)(w.x * inv_cs), cy = (int)(w.y * inv_cs), cz = (int)(w.z * inv_cs);
+    cx = cx < nc ? cx : nc - 1;
+    cy = cy < nc ? cy : nc - 1;
+    cz = cz < nc ? cz : nc - 1;
+    return (cz * nc + cy) * nc + cx;
+}
+
+// the lane's thread index through an empty asm: per-lane addresses derived from it are
+// formed where a phase uses them instead of being hoisted out of the step loop, where
+// one 64-bit address per array would stay live (and spill) for the whole launch
+__device__ __forceinline__ int batch_lane() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+__global__ void __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ order,
               double* __restrict__ x, double* __restrict__ v, double* __restrict__ a,
-              double* __restrict__ vh, double* __restrict__ xref, int* __restrict__ nbr,
+              double* __restrict__ vh, double* __restrict__ xref, uint16_t* __restrict__ nbr,
               int* __restrict__ nnb, const double* __restrict__ ms, int* __restrict__ slot_of,
-              int* __restrict__ cell_of, int* __restrict__ tmp_idx, nbx_batch_params P,
+              nbx_batch_params P,
               const int64_t* __restrict__ step_base, int nsteps, int prologue,
               int* __restrict__ flags) {
     __shared__ float4 spos[BATCH_NMAX];           // positions by slot (cell order)
     __shared__ int ccount[BATCH_MAXCELLS + 1];    // atoms per cell, then fill counters
     __shared__ int cstart[BATCH_MAXCELLS + 1];    // first slot of each cell
+    __shared__ uint16_t tidx[BATCH_NMAX];         // atom indices by slot during a build
+    __shared__ int wsum[BATCH_THREADS / 64];      // wave totals of the cell-count scan
     __shared__ int rebuild;
     const nbx_batch_rep rep = reps[order[blockIdx.x]];
     const int n = rep.n;
     const int tid = threadIdx.x;
     const int64_t nt = P.ntot, off = rep.off;
-    double* X = x + off;
-    double* V = v + off;
-    double* A = a + off;
-    double* VH = vh + off;
-    double* XR = xref + off;
+    // one uniform base pointer per component (X0, X1, X2 = x, y, z of the replica), so
+    // per-lane addressing is a single 32-bit offset shared by every array
+    double* const X0 = x + off, * const X1 = X0 + nt, * const X2 = X1 + nt;
+    double* const V0 = v + off, * const V1 = V0 + nt, * const V2 = V1 + nt;
+    double* const A0 = a + off, * const A1 = A0 + nt, * const A2 = A1 + nt;
+    double* const VH0 = vh + off, * const VH1 = VH0 + nt, * const VH2 = VH1 + nt;
+    double* const XR0 = xref + off, * const XR1 = XR0 + nt, * const XR2 = XR1 + nt;
     const double* M = ms + off;
-    int* NB = nbr + rep.list_off;
+    uint16_t* NB = nbr + rep.list_off;
+    const uint4* NBG = reinterpret_cast<const uint4*>(NB);  // group g of atom i: NBG[g * n + i]
     int* NN = nnb + off;
     int* SL = slot_of + off;
-    int* CI = cell_of + off;
-    int* TI = tmp_idx + off;
     const float Lf = (float)rep.L, invLf = (float)(1.0 / rep.L);
     const int nc = rep.nc;
     const int ncells = nc * nc * nc;
@@ -88,116 +113,203 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
     const int64_t step0 = *step_base;
 
     // Verlet list of every atom, as slots of the cell-ordered LDS positions.  With
-    // nc >= 3: bin (LDS counters), scan, place, then rank-sort each cell's slots by atom
-    // index (so lists are reproducible) and scan the 27 neighbour cells; otherwise slots
-    // are atom indices and every atom scans all atoms.
+    // nc >= 3: bin into LDS counters (the arrival rank and cell of a thread's atoms stay
+    // in registers), scan the counts in parallel, place the atom indices in LDS, rank
+    // each atom among its cell's atoms by index (so slots and lists are reproducible),
+    // then every slot scans the 27 neighbour cells with one periodic shift per cell.
+    // Otherwise slots are atom indices and every atom scans all atoms.  List entries are
+    // packed 8 to a 16-byte group in registers and written a group at a time.
     auto build = [&]() {
+        const int tid = batch_lane();
+        uint4* NBW = reinterpret_cast<uint4*>(NB);
+        // entry `cnt` of atom i is slot u; a full group (or the final partial one) is stored
+        auto emit = [&](int i, int& cnt, uint64_t& lo, uint64_t& hi, unsigned u) {
+            const int e = cnt & 7;
+            const uint64_t val = (uint64_t)u << (16 * (e & 3));
+            lo |= e < 4 ? val : 0;
+            hi |= e < 4 ? 0 : val;
+            if (e == 7) {
+                if (cnt < P.cap)
+                    NBW[(int64_t)(cnt >> 3) * n + i] =
+                        make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+                lo = 0; hi = 0;
+            }
+            ++cnt;
+        };
+        auto finish = [&](int i, int cnt, uint64_t lo, uint64_t hi) {
+            if (cnt > P.cap) { flags[0] = 1; cnt = P.cap; }
+            else if (cnt & 7)
+                NBW[(int64_t)(cnt >> 3) * n + i] =
+                    make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+            NN[i] = cnt;
+        };
         if (nc >= 3) {
             for (int c = tid; c < ncells; c += BATCH_THREADS) ccount[c] = 0;
             __syncthreads();
-            for (int i = tid; i < n; i += BATCH_THREADS) {
-                const float4 w = batch_wrap(X[i], X[nt + i], X[2 * nt + i], rep.L);
-                int cx = (int)(w.x * rep.inv_cs), cy = (int)(w.y * rep.inv_cs),
-                    cz = (int)(w.z * rep.inv_cs);
-                cx = cx < nc ? cx : nc - 1; cy = cy < nc ? cy : nc - 1; cz = cz < nc ? cz : nc - 1;
-                const int c = (cz * nc + cy) * nc + cx;
-                CI[i] = c;
-                atomicAdd(&ccount[c], 1);
+            int code[BATCH_PER_THREAD];  // cell * BATCH_NMAX + arrival rank in the cell
+#pragma unroll
+            for (int m = 0; m < BATCH_PER_THREAD; ++m) {
+                const int i = tid + m * BATCH_THREADS;
+                code[m] = 0;
+                if (i < n) {
+                    const double x0 = X0[i], x1 = X1[i], x2 = X2[i];
+                    XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
+                    const int c = batch_cell(batch_wrap(x0, x1, x2, rep.L), rep.inv_cs, nc);
+                    code[m] = c * BATCH_NMAX + atomicAdd(&ccount[c], 1);
+                }
             }
             __syncthreads();
-            if (tid == 0) {
-                int s0 = 0;
-                for (int c = 0; c < ncells; ++c) { cstart[c] = s0; s0 += ccount[c]; ccount[c] = 0; }
-                cstart[ncells] = s0;
+            {   // exclusive scan of the ncells <= BATCH_THREADS counts: wave scans + wave totals
+                const int lane = tid & 63, w = tid >> 6;
+                const int own = tid < ncells ? ccount[tid] : 0;
+                int v = own;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int u = __shfl_up(v, d, 64);
+                    if (lane >= d) v += u;
+                }
+                if (lane == 63) wsum[w] = v;
+                __syncthreads();
+                int pre = 0;
+                for (int q = 0; q < w; ++q) pre += wsum[q];
+                if (tid < ncells) cstart[tid] = pre + v - own;
+                if (tid == ncells - 1) cstart[ncells] = pre + v;
             }
             __syncthreads();
-            for (int i = tid; i < n; i += BATCH_THREADS) {
-                const int c = CI[i];
-                TI[cstart[c] + atomicAdd(&ccount[c], 1)] = i;
+#pragma unroll
+            for (int m = 0; m < BATCH_PER_THREAD; ++m) {
+                const int i = tid + m * BATCH_THREADS;
+                if (i < n) tidx[cstart[code[m] / BATCH_NMAX] + code[m] % BATCH_NMAX] = (uint16_t)i;
+            }
+            __syncthreads();
+            int slot[BATCH_PER_THREAD];
+#pragma unroll
+            for (int m = 0; m < BATCH_PER_THREAD; ++m) {
+                const int i = tid + m * BATCH_THREADS;
+                slot[m] = 0;
+                if (i < n) {
+                    const int c = code[m] / BATCH_NMAX;
+                    const int u0 = cstart[c], u1 = cstart[c + 1];
+                    int r = 0;
+                    for (int u = u0; u < u1; ++u) r += tidx[u] < i;
+                    slot[m] = u0 + r;
+                    SL[i] = u0 + r;
+                    spos[u0 + r] = batch_wrap(X0[i], X1[i], X2[i], rep.L);
+                }
+            }
+            __syncthreads();
+            // tidx becomes the atom at each slot (the ranks above sort every cell by index)
+#pragma unroll
+            for (int m = 0; m < BATCH_PER_THREAD; ++m) {
+                const int i = tid + m * BATCH_THREADS;
+                if (i < n) tidx[slot[m]] = (uint16_t)i;
             }
             __syncthreads();
             for (int s = tid; s < n; s += BATCH_THREADS) {
-                const int i = TI[s];
-                const int c = CI[i];
-                int r = 0;
-                for (int u = cstart[c]; u < cstart[c + 1]; ++u) r += TI[u] < i;
-                const int slot = cstart[c] + r;
-                SL[i] = slot;
-                spos[slot] = batch_wrap(X[i], X[nt + i], X[2 * nt + i], rep.L);
-            }
-            __syncthreads();
-            for (int i = tid; i < n; i += BATCH_THREADS) {
-                const int si = SL[i];
-                const float4 pi = spos[si];
-                const int c0 = CI[i];
+                const int i = tidx[s];
+                const float4 pi = spos[s];
+                const int c0 = batch_cell(pi, rep.inv_cs, nc);
                 const int cz = c0 / (nc * nc), cy = (c0 / nc) % nc, cx = c0 % nc;
                 int cnt = 0;
-                for (int q = 0; q < 27; ++q) {
-                    const int zz = (cz + q / 9 - 1 + nc) % nc, yy = (cy + (q / 3) % 3 - 1 + nc) % nc,
-                              xx = (cx + q % 3 - 1 + nc) % nc;
-                    const int c = (zz * nc + yy) * nc + xx;
-                    for (int s = cstart[c]; s < cstart[c + 1]; ++s) {
-                        const float4 pj = spos[s];
-                        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                        dx -= Lf * rintf(dx * invLf);
-                        dy -= Lf * rintf(dy * invLf);
-                        dz -= Lf * rintf(dz * invLf);
-                        if (s != si && dx * dx + dy * dy + dz * dz < P.rlist2f) {
-                            if (cnt < P.cap) NB[(int64_t)cnt * n + i] = s;
-                            ++cnt;
+                uint64_t lo = 0, hi = 0;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    int zz = cz + dz;
+                    float oz = 0.0f;
+                    if (zz < 0) { zz += nc; oz = -Lf; } else if (zz >= nc) { zz -= nc; oz = Lf; }
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        int yy = cy + dy;
+                        float oy = 0.0f;
+                        if (yy < 0) { yy += nc; oy = -Lf; } else if (yy >= nc) { yy -= nc; oy = Lf; }
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            int xx = cx + dx;
+                            float ox = 0.0f;
+                            if (xx < 0) { xx += nc; ox = -Lf; } else if (xx >= nc) { xx -= nc; ox = Lf; }
+                            // the neighbour cell's atoms at their image next to this cell
+                            const float qx = pi.x - ox, qy = pi.y - oy, qz = pi.z - oz;
+                            const int c = (zz * nc + yy) * nc + xx;
+                            const int u1 = cstart[c + 1];
+                            // hits of up to 32 candidates as a bit mask (positions read 4 at a
+                            // time ahead of the tests), then the set bits in order
+                            for (int ub = cstart[c]; ub < u1; ub += 32) {
+                                const int nb = u1 - ub < 32 ? u1 - ub : 32;
+                                unsigned hits = 0;
+                                for (int k = 0; k < nb; k += 4) {
+                                    float4 pj[4];
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t)
+                                        pj[t] = spos[k + t < nb ? ub + k + t : ub];
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t) {
+                                        const float ex = qx - pj[t].x, ey = qy - pj[t].y,
+                                                    ez = qz - pj[t].z;
+                                        const bool hit = (k + t < nb) & (ub + k + t != s) &
+                                                         (ex * ex + ey * ey + ez * ez < P.rlist2f);
+                                        hits |= (unsigned)hit << (k + t);
+                                    }
+                                }
+                                while (hits) {
+                                    const int b = __builtin_ctz(hits);
+                                    hits &= hits - 1;
+                                    emit(i, cnt, lo, hi, (unsigned)(ub + b));
+                                }
+                            }
                         }
                     }
                 }
-                if (cnt > P.cap) { flags[0] = 1; cnt = P.cap; }
-                NN[i] = cnt;
-                XR[i] = X[i];
-                XR[nt + i] = X[nt + i];
-                XR[2 * nt + i] = X[2 * nt + i];
+                finish(i, cnt, lo, hi);
             }
         } else {
             for (int i = tid; i < n; i += BATCH_THREADS) {
                 SL[i] = i;
-                spos[i] = batch_wrap(X[i], X[nt + i], X[2 * nt + i], rep.L);
+                const double x0 = X0[i], x1 = X1[i], x2 = X2[i];
+                XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
+                spos[i] = batch_wrap(x0, x1, x2, rep.L);
             }
             __syncthreads();
             for (int i = tid; i < n; i += BATCH_THREADS) {
                 const float4 pi = spos[i];
-                int c = 0;
+                int cnt = 0;
+                uint64_t lo = 0, hi = 0;
                 for (int j = 0; j < n; ++j) {
                     const float4 pj = spos[j];
                     float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     dx -= Lf * rintf(dx * invLf);
                     dy -= Lf * rintf(dy * invLf);
                     dz -= Lf * rintf(dz * invLf);
-                    if (j != i && dx * dx + dy * dy + dz * dz < P.rlist2f) {
-                        if (c < P.cap) NB[(int64_t)c * n + i] = j;
-                        ++c;
-                    }
+                    if (j != i && dx * dx + dy * dy + dz * dz < P.rlist2f) emit(i, cnt, lo, hi, (unsigned)j);
                 }
-                if (c > P.cap) { flags[0] = 1; c = P.cap; }
-                NN[i] = c;
-                XR[i] = X[i];
-                XR[nt + i] = X[nt + i];
-                XR[2 * nt + i] = X[2 * nt + i];
+                finish(i, cnt, lo, hi);
             }
         }
         if (tid == 0) atomicAdd(&flags[1], 1);
     };
-    // forces of the current positions, then (do_kick) the end of step `step`
+    // forces of the current positions, then (do_kick) the end of step `step`.  The next
+    // group of 8 slots and the next atom's count, slot and first group are loaded one
+    // group ahead; entries past the count are masked to the atom's own slot (in range)
+    // and skipped.
     auto force = [&](int do_kick, int64_t step) {
-        for (int i = tid; i < n; i += BATCH_THREADS) {
-            const float4 pi = spos[SL[i]];
+        const int tid = batch_lane();
+        int i = tid;
+        int cnt_n = 0, sl_n = 0;
+        uint4 g_n = make_uint4(0u, 0u, 0u, 0u);
+        if (i < n) { cnt_n = NN[i]; sl_n = SL[i]; g_n = NBG[i]; }
+        for (; i < n; i += BATCH_THREADS) {
+            const int cnt = cnt_n, si = sl_n;
+            uint4 cur = g_n;
+            const int inx = i + BATCH_THREADS;
+            if (inx < n) { cnt_n = NN[inx]; sl_n = SL[inx]; g_n = NBG[inx]; }
+            const double mi = M[i];
+            double vh3[3] = {0.0, 0.0, 0.0};
+            if (do_kick) { vh3[0] = VH0[i]; vh3[1] = VH1[i]; vh3[2] = VH2[i]; }
+            const float4 pi = spos[si];
             nbx_kahan fx, fy, fz;
             fx.init(); fy.init(); fz.init();
-            const int cnt = NN[i];
-            for (int k = 0; k < cnt; ++k) {
-                const float4 pj = spos[NB[(int64_t)k * n + i]];
+            auto pair = [&](const float4 pj, bool ok) {
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 dx -= Lf * rintf(dx * invLf);
                 dy -= Lf * rintf(dy * invLf);
                 dz -= Lf * rintf(dz * invLf);
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < P.R2) {
+                if (ok && r2 < P.R2) {
                     const float inv = __builtin_amdgcn_rcpf(r2);
                     const float q = P.s2 * inv;
                     const float s6 = q * q * q;
@@ -206,17 +318,35 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                     fy.add(f * dy);
                     fz.add(f * dz);
                 }
+            };
+            // four slots of a group: positions read together, then the pairs in order
+            auto quad = [&](unsigned w0, unsigned w1, int e0, int rem) {
+                const unsigned sl[4] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu, w1 >> 16};
+                float4 pj[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) pj[t] = spos[e0 + t < rem ? sl[t] : (unsigned)si];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) pair(pj[t], e0 + t < rem);
+            };
+            const int ng = (cnt + 7) >> 3;
+            for (int g = 0; g < ng; ++g) {
+                uint4 nxt = cur;
+                if (g + 1 < ng) nxt = NBG[(int64_t)(g + 1) * n + i];
+                const int rem = cnt - 8 * g;
+                quad(cur.x, cur.y, 0, rem);
+                quad(cur.z, cur.w, 4, rem);
+                cur = nxt;
             }
-            const double c = P.c24eps / M[i];
+            const double c = P.c24eps / mi;
             const double ax = c * fx.value(), ay = c * fy.value(), az = c * fz.value();
-            A[i] = ax;
-            A[nt + i] = ay;
-            A[2 * nt + i] = az;
+            A0[i] = ax;
+            A1[i] = ay;
+            A2[i] = az;
             if (!do_kick) continue;
-            double v3[3] = {VH[i] + dt * (0.5 * ax), VH[nt + i] + dt * (0.5 * ay),
-                            VH[2 * nt + i] + dt * (0.5 * az)};
+            double v3[3] = {vh3[0] + dt * (0.5 * ax), vh3[1] + dt * (0.5 * ay),
+                            vh3[2] + dt * (0.5 * az)};
             if (rep.therm == NBX_THERM_LANGEVIN) {
-                const double sig = rep.c2 * sqrt(P.kb * rep.T / M[i]);
+                const double sig = rep.c2 * sqrt(P.kb * rep.T / mi);
                 double g[3];
                 nbx_rng_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_B, g);
                 for (int k = 0; k < 3; ++k) v3[k] = rep.c1 * v3[k] + sig * g[k];
@@ -227,9 +357,9 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 nbx_rng_normals3(rep.key, i, step, NBX_RNG_ANDERSEN, g);
                 for (int k = 0; k < 3; ++k) v3[k] = vdev * g[k];
             }
-            V[i] = v3[0];
-            V[nt + i] = v3[1];
-            V[2 * nt + i] = v3[2];
+            V0[i] = v3[0];
+            V1[i] = v3[1];
+            V2[i] = v3[2];
         }
     };
 
@@ -243,27 +373,51 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
         const int64_t step = step0 + s;
         if (tid == 0) rebuild = 0;
         __syncthreads();
-        for (int i = tid; i < n; i += BATCH_THREADS) {
-            double vx = V[i], vy = V[nt + i], vz = V[2 * nt + i];
-            if (rep.therm == NBX_THERM_LANGEVIN) {
-                const double sig = rep.c2 * sqrt(P.kb * rep.T / M[i]);
+        // drift (and the first Langevin half step).  Without Langevin the next atom's
+        // state is loaded while the current one is computed; with it, the current atom's
+        // loads are in flight during the random numbers (prefetching as well would
+        // exceed the register budget of four waves per SIMD)
+        struct dstate { double v[3], a[3], x[3], r[3], m; int sl; };
+        auto dload = [&](int j, dstate& d) {
+            d.v[0] = V0[j]; d.v[1] = V1[j]; d.v[2] = V2[j];
+            d.a[0] = A0[j]; d.a[1] = A1[j]; d.a[2] = A2[j];
+            d.x[0] = X0[j]; d.x[1] = X1[j]; d.x[2] = X2[j];
+            d.r[0] = XR0[j]; d.r[1] = XR1[j]; d.r[2] = XR2[j];
+            d.m = M[j];
+            d.sl = SL[j];
+        };
+        auto dstore = [&](int i, const dstate& d, double vx, double vy, double vz) {
+            const double ax = d.a[0], ay = d.a[1], az = d.a[2];
+            const double px = d.x[0] + dt * vx + dtsq * (0.5 * ax);
+            const double py = d.x[1] + dt * vy + dtsq * (0.5 * ay);
+            const double pz = d.x[2] + dt * vz + dtsq * (0.5 * az);
+            X0[i] = px; X1[i] = py; X2[i] = pz;
+            VH0[i] = vx + dt * (0.5 * ax);
+            VH1[i] = vy + dt * (0.5 * ay);
+            VH2[i] = vz + dt * (0.5 * az);
+            spos[d.sl] = batch_wrap(px, py, pz, rep.L);
+            const double ex = px - d.r[0], ey = py - d.r[1], ez = pz - d.r[2];
+            if (ex * ex + ey * ey + ez * ez > P.half_skin2) rebuild = 1;
+        };
+        const int t0 = batch_lane();
+        if (rep.therm == NBX_THERM_LANGEVIN) {
+            for (int i = t0; i < n; i += BATCH_THREADS) {
+                dstate d;
+                dload(i, d);
                 double g[3];
                 nbx_rng_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_A, g);
-                vx = rep.c1 * vx + sig * g[0];
-                vy = rep.c1 * vy + sig * g[1];
-                vz = rep.c1 * vz + sig * g[2];
+                const double sig = rep.c2 * sqrt(P.kb * rep.T / d.m);
+                dstore(i, d, rep.c1 * d.v[0] + sig * g[0], rep.c1 * d.v[1] + sig * g[1],
+                       rep.c1 * d.v[2] + sig * g[2]);
             }
-            const double ax = A[i], ay = A[nt + i], az = A[2 * nt + i];
-            const double px = X[i] + dt * vx + dtsq * (0.5 * ax);
-            const double py = X[nt + i] + dt * vy + dtsq * (0.5 * ay);
-            const double pz = X[2 * nt + i] + dt * vz + dtsq * (0.5 * az);
-            X[i] = px; X[nt + i] = py; X[2 * nt + i] = pz;
-            VH[i] = vx + dt * (0.5 * ax);
-            VH[nt + i] = vy + dt * (0.5 * ay);
-            VH[2 * nt + i] = vz + dt * (0.5 * az);
-            spos[SL[i]] = batch_wrap(px, py, pz, rep.L);
-            const double ex = px - XR[i], ey = py - XR[nt + i], ez = pz - XR[2 * nt + i];
-            if (ex * ex + ey * ey + ez * ez > P.half_skin2) rebuild = 1;
+        } else {
+            dstate dn;
+            if (t0 < n) dload(t0, dn);
+            for (int i = t0; i < n; i += BATCH_THREADS) {
+                const dstate d = dn;
+                if (i + BATCH_THREADS < n) dload(i + BATCH_THREADS, dn);
+                dstore(i, d, d.v[0], d.v[1], d.v[2]);
+            }
         }
         __syncthreads();
         if (rebuild) {
@@ -332,14 +486,13 @@ k_batch_energies(const nbx_batch_rep* __restrict__ reps, const double* __restric
 }
 
 void batch_launch_steps(hipStream_t s, int nblocks, const void* reps, const int* order,
-                        double* x, double* v, double* a, double* vh, double* xref, int* nbr,
-                        int* nnb, const double* ms, int* scratch3, int64_t ntot,
+                        double* x, double* v, double* a, double* vh, double* xref, uint16_t* nbr,
+                        int* nnb, const double* ms, int* slot_of,
                         const nbx_batch_params& P, const int64_t* step_base, int nsteps,
                         int prologue, int* flags) {
     hipLaunchKernelGGL(k_batch_steps, dim3(nblocks), dim3(BATCH_THREADS), 0, s,
                        (const nbx_batch_rep*)reps, order, x, v, a, vh, xref, nbr, nnb, ms,
-                       scratch3, scratch3 + ntot, scratch3 + 2 * ntot, P, step_base, nsteps,
-                       prologue, flags);
+                       slot_of, P, step_base, nsteps, prologue, flags);
 }
 
 void batch_launch_energies(hipStream_t s, int nrep, const void* reps, const double* x,
@@ -396,10 +549,10 @@ void batch_fill_params(void* dst, double eps, double sigma, double R, double rli
 size_t batch_params_size() { return sizeof(nbx_batch_params); }
 
 void batch_launch_steps_raw(hipStream_t s, int nblocks, const void* reps, const int* order,
-                            double* x, double* v, double* a, double* vh, double* xref, int* nbr,
-                            int* nnb, const double* ms, int* scratch3, const void* params,
+                            double* x, double* v, double* a, double* vh, double* xref, uint16_t* nbr,
+                            int* nnb, const double* ms, int* slot_of, const void* params,
                             const int64_t* step_base, int nsteps, int prologue, int* flags) {
     const nbx_batch_params& P = *(const nbx_batch_params*)params;
-    batch_launch_steps(s, nblocks, reps, order, x, v, a, vh, xref, nbr, nnb, ms, scratch3, P.ntot,
-                       P, step_base, nsteps, prologue, flags);
+    batch_launch_steps(s, nblocks, reps, order, x, v, a, vh, xref, nbr, nnb, ms, slot_of, P,
+                       step_base, nsteps, prologue, flags);
 }
