@@ -144,7 +144,19 @@ int nbx_accelerations(nbx_ctx* ctx, int32_t term, const double* u, int64_t i_beg
  * On return the context state is the final state. */
 int nbx_vv_run(nbx_ctx* ctx, double dt, int64_t nsteps, uint64_t seed, int64_t save_every,
                double* u_frames, double* v_frames);
-/* Steps taken by all nbx_vv_run calls since the last set_state (solution.stats.naccept). */
+/* Euler-Maruyama run of the Langevin SDE that the reference builds for a potential
+ * system with a LangevinThermostat (SciMLBase.SDEProblem, src/nbody_to_ode.jl:567-598,
+ * solved by StochasticDiffEq EM through run_simulation, src/nbody_simulation_result.jl:
+ * 480-487): with a = f(x) the potential accelerations,
+ *   x' = x + dt v;  v' = v + dt (a - gamma v) + sigma sqrt(dt) xi,
+ *   sigma = sqrt(2 gamma kb T / m_1)  (the first body's mass, for every atom).
+ * Needs the Langevin thermostat (T, gamma); water systems are not supported.  xi comes
+ * from Philox4x32-10 keyed by `seed` with counter (atom, absolute step); steps count
+ * into nbx_steps_taken like nbx_vv_run.  Frames and the final state as nbx_vv_run. */
+int nbx_em_run(nbx_ctx* ctx, double dt, int64_t nsteps, uint64_t seed, int64_t save_every,
+               double* u_frames, double* v_frames);
+/* Steps taken by all nbx_vv_run / nbx_em_run calls since the last set_state
+ * (solution.stats.naccept). */
 int64_t nbx_steps_taken(const nbx_ctx* ctx);
 
 /* Energies of the current state, in FP64 on the device:

@@ -606,6 +606,69 @@ extern "C" int nbx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed,
 }
 
 // ---------------------------------------------------------------------------
+// Euler-Maruyama (the reference's Langevin SDE path)
+// ---------------------------------------------------------------------------
+
+static int ref_em(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every,
+                  double* uf, double* vf, int64_t* nframes) {
+    hipStream_t s = c->stream;
+    int rc;
+    if (!c->accel_valid) {
+        if ((rc = ref_rhs(c, c->d_x, c->d_v, c->d_a, c->d_scal + NBX_SCAL_XI))) return rc;
+        c->accel_valid = true;
+    }
+    const double sigma = std::sqrt(2 * c->therm_p * c->kb * c->therm_T / c->h_ms[0]);
+    for (int64_t k = 1; k <= nsteps; ++k) {
+        const int64_t step = c->steps_taken + 1;
+        ref64_em(s, c->d_x, c->d_v, c->d_a, c->n, dt, c->therm_p, sigma, seed, step);
+        if ((rc = ref_rhs(c, c->d_x, c->d_v, c->d_a, c->d_scal + NBX_SCAL_XI))) return rc;
+        NBX_HIP(c, hipGetLastError());
+        c->steps_taken = step;
+        if (save_every > 0 && uf && k % save_every == 0) {
+            if ((rc = save_frame(c, *nframes, uf, vf))) return rc;
+            ++*nframes;
+        }
+    }
+    return NBX_OK;
+}
+
+extern "C" int nbx_em_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed,
+                          int64_t save_every, double* u_frames, double* v_frames) {
+    if (!c) return NBX_ERR_INVALID;
+    if (!c->state_set) return nbx_fail(c, NBX_ERR_INVALID, "no state set");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return nbx_fail(c, NBX_ERR_INVALID, "dt must be > 0");
+    if (nsteps < 0) return nbx_fail(c, NBX_ERR_INVALID, "nsteps < 0");
+    if (save_every > 0 && (!u_frames || !v_frames))
+        return nbx_fail(c, NBX_ERR_INVALID, "frames requested but frame buffers are NULL");
+    if (c->therm != NBX_THERM_LANGEVIN)
+        return nbx_fail(c, NBX_ERR_INVALID,
+                        "Euler-Maruyama integrates the Langevin SDE: set the Langevin thermostat");
+    if (c->sys == NBX_SYS_WATER)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED,
+                        "the Langevin SDE is provided for potential systems only");
+    int rc = check_config(c);
+    if (rc) return rc;
+    if (c->n == 0) { c->steps_taken += nsteps; return NBX_OK; }
+    int64_t nframes = 0;
+    if (save_every > 0) {
+        if ((rc = save_frame(c, 0, u_frames, v_frames))) return rc;
+        nframes = 1;
+    }
+    if (c->prec == NBX_PREC_MIXED) {
+        std::string why;
+        if (!mx_supported(c, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+        c->em = 1;
+        rc = mx_vv(c, dt, nsteps, seed, save_every, u_frames, v_frames, &nframes);
+        c->em = 0;
+    } else {
+        rc = ref_em(c, dt, nsteps, seed, save_every, u_frames, v_frames, &nframes);
+    }
+    if (rc) return rc;
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
+    return check_finite_state(c);
+}
+
+// ---------------------------------------------------------------------------
 // Energies, temperature, RDF
 // ---------------------------------------------------------------------------
 

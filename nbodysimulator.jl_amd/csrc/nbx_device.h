@@ -82,7 +82,7 @@ __device__ __forceinline__ double nbx_u01(uint32_t a, uint32_t b) {
 }
 
 // RNG streams; counter = (atom, step, stream, sub), key = seed
-enum { NBX_RNG_ANDERSEN = 0, NBX_RNG_LANGEVIN_A = 1, NBX_RNG_LANGEVIN_B = 2 };
+enum { NBX_RNG_ANDERSEN = 0, NBX_RNG_LANGEVIN_A = 1, NBX_RNG_LANGEVIN_B = 2, NBX_RNG_EM = 3 };
 
 __device__ __forceinline__ double nbx_rng_uniform(uint64_t seed, int64_t atom, int64_t step,
                                                   uint32_t stream) {

@@ -60,12 +60,14 @@ struct nbx_graph_key {
     const void* pos = nullptr;
     const void* part = nullptr;
     int cap = 0, ccap = 0, ncell = 0, lanes = 0, steps = 0, chunks = 0, tpt = 0;
+    int em = 0;               // Euler-Maruyama steps instead of velocity Verlet
     bool operator==(const nbx_graph_key& o) const {
         return valid && o.valid && version == o.version && dt == o.dt && seed == o.seed &&
                nbr == o.nbr && nnb == o.nnb && cell_count == o.cell_count &&
                cell_atoms == o.cell_atoms && xref == o.xref && flags == o.flags && pos == o.pos &&
                part == o.part && cap == o.cap && ccap == o.ccap && ncell == o.ncell &&
-               lanes == o.lanes && steps == o.steps && chunks == o.chunks && tpt == o.tpt;
+               lanes == o.lanes && steps == o.steps && chunks == o.chunks && tpt == o.tpt &&
+               em == o.em;
     }
 };
 
@@ -86,6 +88,7 @@ struct nbx_ctx {
     bool spcfw_set = false;
     int therm = NBX_THERM_NONE;
     double therm_T = 0, therm_p = 0;
+    int em = 0;               // set while nbx_em_run steps the state (Euler-Maruyama)
     double skin = 0.0;        // user skin (when skin_set), else 0.12 x the largest cutoff
     bool skin_set = false;
     // device state
@@ -174,6 +177,8 @@ void ref64_drift(hipStream_t s, double* x, const double* v, const double* a, dou
                  int64_t n, double dt);
 void ref64_kick(hipStream_t s, double* v, const double* vh, const double* a, int64_t n,
                 double dt);
+void ref64_em(hipStream_t s, double* x, double* v, const double* a, int64_t n, double dt,
+              double gamma, double sigma, uint64_t seed, int64_t step);
 void ref64_langevin_half(hipStream_t s, double* v, const double* ms, int64_t n, double kb,
                          double T, double gamma, double dt, uint64_t seed, int64_t step,
                          uint32_t stream);

@@ -187,7 +187,13 @@ mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t se
     kp.seed = seed;
     kp.m0 = c->n > 0 ? c->h_ms[0] : 1.0;
     kp.d_T = c->d_scal;  // [0]
-    if (c->therm == NBX_THERM_LANGEVIN) {
+    if (c->em) {
+        // sigma = sqrt(2 gamma kb T / m_1) (src/nbody_to_ode.jl:592)
+        kp.em = 1;
+        kp.em_gamma = c->therm_p;
+        kp.em_sigma = std::sqrt(2 * c->therm_p * c->kb * c->therm_T / kp.m0);
+        kp.em_sqrt_dt = std::sqrt(dt);
+    } else if (c->therm == NBX_THERM_LANGEVIN) {
         kp.c1 = std::exp(-c->therm_p * (0.5 * dt));
         kp.c2 = std::sqrt(1.0 - kp.c1 * kp.c1);
     } else if (c->therm == NBX_THERM_ANDERSEN) {
@@ -445,6 +451,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         key.part = c->d_part;
         key.chunks = c->grav_chunks;
         key.tpt = c->grav_tpt_set;
+        key.em = c->em;
         key.cap = c->nl.cap;
         key.ccap = c->nl.ccap;
         key.ncell = c->nl.ncell;

@@ -39,6 +39,10 @@ struct mx_kick_params {
     const int64_t* step_base;  // if set, step = *step_base + step_off (graph replays)
     int64_t step_off;
     const double* d_T;    // Berendsen: temperature of the half-kicked velocities
+    // Euler-Maruyama steps of the Langevin SDE (src/nbody_to_ode.jl:567-598) instead of
+    // velocity Verlet: x' = x + dt v, v' = v + dt (a - gamma v) + sigma sqrt(dt) xi
+    int em;
+    double em_gamma, em_sigma, em_sqrt_dt;
 };
 
 // Buffers of one fused step launch
@@ -76,11 +80,18 @@ __device__ __forceinline__ void mx_resolve_step(mx_kick_params& kp) {
     if (kp.step_base != nullptr) kp.step = *kp.step_base + kp.step_off;
 }
 
-// v = vh + dt (a/2), then the Langevin B half-step of step kp.step
+// v = vh + dt (a/2), then the Langevin B half-step of step kp.step (Euler-Maruyama:
+// v = vh, the velocity the previous drift computed)
 __device__ __forceinline__ void mx_kick_atom(int64_t i, int64_t n, double ax, double ay,
                                              double az, const double* __restrict__ vh,
                                              const double* __restrict__ ms,
                                              const mx_kick_params& kp, double* v3) {
+    if (kp.em) {  // Euler-Maruyama: vh already holds v of the step being finished
+        v3[0] = vh[i];
+        v3[1] = vh[n + i];
+        v3[2] = vh[2 * n + i];
+        return;
+    }
     v3[0] = vh[i] + kp.dt * (0.5 * ax);
     v3[1] = vh[n + i] + kp.dt * (0.5 * ay);
     v3[2] = vh[2 * n + i] + kp.dt * (0.5 * az);
@@ -111,6 +122,44 @@ __device__ __forceinline__ double mx_wrap(double p, double L) {
     return w >= L ? w - L : w;
 }
 
+// wrapped copy of x_{k+1} and the skin test against the positions of the last build
+__device__ __forceinline__ void mx_store_position(int64_t i, int64_t n, double px, double py,
+                                                  double pz, double mi, const mx_io& io) {
+    if (io.pos_kind == 1) {
+        const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
+        ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, io.qs != nullptr ? io.qs[i] : 0.0);
+    } else if (io.pos_kind == 2) {
+        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)mi);
+    }
+    if (io.xref != nullptr) {
+        const double ex = px - io.xref[i], ey = py - io.xref[n + i], ez = pz - io.xref[2 * n + i];
+        if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
+    }
+}
+
+// Euler-Maruyama step kp.step + 1 of the Langevin SDE (src/nbody_to_ode.jl:575-596) from
+// (x, v, a = f(x)): x' = x + dt v and v' = v + dt (a - gamma v) + sigma (sqrt(dt) xi),
+// sigma = sqrt(2 gamma kb T / m_1) for every atom; v' goes to vh (the next step's kick
+// copies it to v).  Returns 0 (no thermostat temperature is needed).
+__device__ __forceinline__ double mx_em_atom(int64_t i, int64_t n, double vx, double vy,
+                                             double vz, double ax, double ay, double az,
+                                             const mx_kick_params& kp, const mx_io& io) {
+    const double dt = kp.dt;
+    const double px = io.x_in[i] + dt * vx;
+    const double py = io.x_in[n + i] + dt * vy;
+    const double pz = io.x_in[2 * n + i] + dt * vz;
+    io.x_out[i] = px;
+    io.x_out[n + i] = py;
+    io.x_out[2 * n + i] = pz;
+    double g[3];
+    nbx_rng_normals3(kp.seed, i, kp.step + 1, NBX_RNG_EM, g);
+    io.vh[i] = vx + dt * (ax - kp.em_gamma * vx) + kp.em_sigma * (kp.em_sqrt_dt * g[0]);
+    io.vh[n + i] = vy + dt * (ay - kp.em_gamma * vy) + kp.em_sigma * (kp.em_sqrt_dt * g[1]);
+    io.vh[2 * n + i] = vz + dt * (az - kp.em_gamma * vz) + kp.em_sigma * (kp.em_sqrt_dt * g[2]);
+    mx_store_position(i, n, px, py, pz, io.ms[i], io);
+    return 0.0;
+}
+
 // Start of step kp.step + 1 for atom i from v (after the previous step's end) and a:
 // Langevin A half-step in registers, drift, half kick, wrapped copy, bins, skin test.
 // Returns m vh^2 (the Berendsen temperature input).
@@ -119,6 +168,7 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
                                                 const mx_kick_params& kp, const mx_io& io) {
     double vx = v3[0], vy = v3[1], vz = v3[2];
     const double mi = io.ms[i];
+    if (kp.em) return mx_em_atom(i, n, vx, vy, vz, ax, ay, az, kp, io);
     if (kp.therm == NBX_THERM_LANGEVIN) {
         const double sig = kp.c2 * sqrt(kp.kb * kp.T / mi);
         double g[3];
@@ -138,16 +188,7 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
     io.vh[i] = hx;
     io.vh[n + i] = hy;
     io.vh[2 * n + i] = hz;
-    if (io.pos_kind == 1) {
-        const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
-        ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, io.qs != nullptr ? io.qs[i] : 0.0);
-    } else if (io.pos_kind == 2) {
-        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)mi);
-    }
-    if (io.xref != nullptr) {
-        const double ex = px - io.xref[i], ey = py - io.xref[n + i], ez = pz - io.xref[2 * n + i];
-        if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
-    }
+    mx_store_position(i, n, px, py, pz, mi, io);
     return mi * (hx * hx + hy * hy + hz * hz);
 }
 

@@ -508,6 +508,30 @@ void ref64_langevin_half(hipStream_t s, double* v, const double* ms, int64_t n, 
                        kb, T, c1, c2, seed, step, stream);
 }
 
+// Euler-Maruyama step of the Langevin SDE (src/nbody_to_ode.jl:567-598), one thread per
+// atom, from the state (x, v) and a = f(x): x' = x + dt v and
+// v' = v + dt (a - gamma v) + sigma (sqrt(dt) xi), sigma = sqrt(2 gamma kb T / m_1)
+__global__ void k_ref_em(double* __restrict__ x, double* __restrict__ v,
+                         const double* __restrict__ a, int64_t n, double dt, double gamma,
+                         double sigma, double sqrt_dt, uint64_t seed, int64_t step) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double g[3];
+    nbx_rng_normals3(seed, i, step, NBX_RNG_EM, g);
+    for (int k = 0; k < 3; ++k) {
+        const double vk = v[k * n + i];
+        x[k * n + i] = x[k * n + i] + dt * vk;
+        v[k * n + i] = vk + dt * (a[k * n + i] - gamma * vk) + sigma * (sqrt_dt * g[k]);
+    }
+}
+
+void ref64_em(hipStream_t s, double* x, double* v, const double* a, int64_t n, double dt,
+              double gamma, double sigma, uint64_t seed, int64_t step) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ref_em, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, x, v, a, n, dt, gamma,
+                       sigma, sqrt(dt), seed, step);
+}
+
 // Andersen collisions (src/nbody_simulation_result.jl:511-540), one thread per body
 __global__ void k_ref_andersen(double* __restrict__ v, const double* __restrict__ ms, int64_t n,
                                int water, double kb, double T, double prob, uint64_t seed,

@@ -31,7 +31,7 @@ import NBodySimulator: get_accelerating_function, run_simulation
 import SciMLBase
 using RecursiveArrayTools: ArrayPartition
 
-export MI355XVerlet, MI355XPotential
+export MI355XVerlet, MI355XEM, MI355XPotential
 
 const libnbx = get(ENV, "NBX_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libnbx.so"))
 
@@ -49,6 +49,20 @@ for bit; `:mixed` evaluates pair forces in FP32 (Kahan sums) with FP64 state and
 cell/Verlet lists.
 """
 Base.@kwdef struct MI355XVerlet
+    precision::Symbol = :fp64
+    seed::UInt64 = 0
+    skin::Union{Nothing, Float64} = nothing
+end
+
+"""
+    MI355XEM(; precision = :fp64, seed = 0, skin = nothing)
+
+Euler-Maruyama on the GPU for the Langevin SDE of a potential system with a
+`LangevinThermostat` -- the `SDEProblem(sim)` + `EM()` path of the reference
+(src/nbody_to_ode.jl:567-598): x' = x + dt v, v' = v + dt (a - gamma v) + sigma sqrt(dt) xi,
+sigma = sqrt(2 gamma kb T / m_1).
+"""
+Base.@kwdef struct MI355XEM
     precision::Symbol = :fp64
     seed::UInt64 = 0
     skin::Union{Nothing, Float64} = nothing
@@ -186,6 +200,42 @@ function run_simulation(sim::NBodySimulation, alg::MI355XVerlet; dt::Real,
     ts = [t1 + k * every * dt for k in 0:(nframes - 1)]
     us = [ArrayPartition(vf[:, :, k], uf[:, :, k]) for k in 1:nframes]
     prob = SciMLBase.SecondOrderODEProblem(sim)
+    sol = SciMLBase.build_solution(prob, alg, ts, us)
+    sol.stats.naccept = nsteps
+    return SimulationResult(sol, sim)
+end
+
+"""
+    run_simulation(sim, alg::MI355XEM; dt, save_everystep = true)
+
+GPU replacement of `run_simulation(sim, EM(); dt)` for a potential system with a
+`LangevinThermostat`: the frames are 3 x 2n matrices `[u v]`, as the reference's
+`SDEProblem` solution holds them, so `get_position` / `get_velocity` keep working.
+"""
+function run_simulation(sim::NBodySimulation, alg::MI355XEM; dt::Real,
+                        save_everystep::Bool = true)
+    sim.thermostat isa LangevinThermostat ||
+        throw(ArgumentError("MI355XEM integrates the Langevin SDE: use a LangevinThermostat"))
+    ctx = build_context(sim, alg.precision)
+    alg.skin === nothing ||
+        check(ctx, ccall((:nbx_set_skin, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, alg.skin))
+    u0, v0, _ = NBodySimulator.gather_bodies_initial_coordinates(sim)
+    n = ctx.n
+    u0 = Matrix{Float64}(u0[:, 1:n]); v0 = Matrix{Float64}(v0[:, 1:n])
+    check(ctx, ccall((:nbx_set_state, libnbx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.ptr, u0, v0))
+    t1, t2 = sim.tspan
+    nsteps = round(Int, (t2 - t1) / dt)
+    every = save_everystep ? 1 : max(nsteps, 1)
+    nframes = 1 + nsteps ÷ every
+    uf = Array{Float64}(undef, 3, n, nframes)
+    vf = Array{Float64}(undef, 3, n, nframes)
+    check(ctx, ccall((:nbx_em_run, libnbx), Cint,
+                     (Ptr{Cvoid}, Cdouble, Int64, UInt64, Int64, Ptr{Float64}, Ptr{Float64}),
+                     ctx.ptr, dt, nsteps, alg.seed, every, uf, vf))
+    ts = [t1 + k * every * dt for k in 0:(nframes - 1)]
+    us = [hcat(uf[:, :, k], vf[:, :, k]) for k in 1:nframes]
+    prob = SciMLBase.SDEProblem(sim)
     sol = SciMLBase.build_solution(prob, alg, ts, us)
     sol.stats.naccept = nsteps
     return SimulationResult(sol, sim)

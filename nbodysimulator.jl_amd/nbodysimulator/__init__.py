@@ -2,10 +2,12 @@
 
 Same names, fields and argument meaning as the Julia package (module export list,
 src/NBodySimulator.jl:26-36); the force evaluation, the velocity-Verlet step, the
-thermostats, energies, temperature and rdf run as HIP kernels on the GPU.
+thermostats (and the Euler-Maruyama step of the Langevin SDE), energies, temperature
+and rdf run as HIP kernels on the GPU.
 """
 from ._lib import LIB_PATH, Batch, Context, Model, NBXError, device_count, load
-from .simulation import (MI355XVerlet, SimulationResult, Solution, VelocityVerlet,
+from .simulation import (EM, EulerMaruyama, MI355XVerlet, SimulationResult, Solution,
+                         VelocityVerlet,
                          build_context, get_accelerating_function,
                          get_group_accelerating_function, get_position, get_velocity,
                          gravitational_potential_energy, initial_energy, kinetic_energy,

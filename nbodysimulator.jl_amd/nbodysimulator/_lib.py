@@ -58,6 +58,7 @@ SIGNATURES = {
     "nbx_rhs": (C.c_int, [_P, _DP, _DP, _DP]),
     "nbx_accelerations": (C.c_int, [_P, _I32, _DP, _I64, _I64, _DP]),
     "nbx_vv_run": (C.c_int, [_P, _D, _I64, _U64, _I64, _DP, _DP]),
+    "nbx_em_run": (C.c_int, [_P, _D, _I64, _U64, _I64, _DP, _DP]),
     "nbx_steps_taken": (_I64, [_P]),
     "nbx_energies": (C.c_int, [_P, _DP, _DP, _DP]),
     "nbx_temperature": (C.c_int, [_P, _DP]),
@@ -269,6 +270,17 @@ class Context:
             uf = np.empty((nframes, self.n, 3))
             vf = np.empty((nframes, self.n, 3))
         self._check(self._lib.nbx_vv_run(self._h, float(dt), int(nsteps), int(seed),
+                                          int(save_every), dptr(uf), dptr(vf)))
+        return uf, vf
+
+    def em_run(self, dt: float, nsteps: int, seed: int = 0, save_every: int = 0):
+        """Euler-Maruyama steps of the Langevin SDE (nbx_em_run); frames as vv_run."""
+        nframes = 1 + nsteps // save_every if save_every > 0 else 0
+        uf = vf = None
+        if nframes:
+            uf = np.empty((nframes, self.n, 3))
+            vf = np.empty((nframes, self.n, 3))
+        self._check(self._lib.nbx_em_run(self._h, float(dt), int(nsteps), int(seed),
                                           int(save_every), dptr(uf), dptr(vf)))
         return uf, vf
 

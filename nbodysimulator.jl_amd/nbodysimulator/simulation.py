@@ -37,6 +37,20 @@ MI355XVerlet = VelocityVerlet
 
 
 @dataclass
+class EM:
+    """Euler-Maruyama (StochasticDiffEq.EM) for the Langevin SDE of a potential system
+    with a LangevinThermostat (SciMLBase.SDEProblem, src/nbody_to_ode.jl:567-598),
+    executed on the GPU: x' = x + dt v, v' = v + dt (a - gamma v) + sigma sqrt(dt) xi with
+    sigma = sqrt(2 gamma kb T / m_1).  precision and skin as for VelocityVerlet."""
+    precision: str = "fp64"
+    seed: int = 0
+    skin: Optional[float] = None
+
+
+EulerMaruyama = EM
+
+
+@dataclass
 class _Stats:
     naccept: int = 0
 
@@ -194,14 +208,16 @@ def run_simulation(s: NBodySimulation, alg=None, *, dt: float = None,
 
     As in the reference, every step is saved unless save_everystep=False (then the
     first and last states only).  Thermostats: Andersen and Berendsen as in the
-    reference; Langevin as Ornstein-Uhlenbeck half steps around each VV step (the
-    reference integrates an SDE with Euler-Maruyama instead)."""
+    reference; Langevin as Ornstein-Uhlenbeck half steps around each VV step.  With
+    alg = EM() the Langevin SDE is integrated with Euler-Maruyama, as the reference's
+    SDEProblem path does."""
     if alg is None:
         alg = VelocityVerlet()
-    if not isinstance(alg, VelocityVerlet):
-        raise NotImplementedError("the GPU path integrates with VelocityVerlet / MI355XVerlet")
+    if not isinstance(alg, (VelocityVerlet, EM)):
+        raise NotImplementedError("the GPU path integrates with VelocityVerlet / MI355XVerlet "
+                                  "or EM (the Langevin SDE)")
     if dt is None:
-        raise ValueError("VelocityVerlet needs a fixed dt")
+        raise ValueError(f"{type(alg).__name__} needs a fixed dt")
     prec = precision or alg.precision
     rng_seed = alg.seed if seed is None else seed
     ctx = context or build_context(s, prec)
@@ -211,23 +227,24 @@ def run_simulation(s: NBodySimulation, alg=None, *, dt: float = None,
     ctx.set_state(u0, v0)
     t1, t2 = float(s.tspan[0]), float(s.tspan[1])
     nsteps, dt, last = _step_plan(t1, t2, float(dt))
+    step = ctx.em_run if isinstance(alg, EM) else ctx.vv_run
     save = True if save_everystep is None else bool(save_everystep)
     ts = [t1]
     us = [u0.copy()]
     vs = [v0.copy()]
     if save:
         if nsteps:
-            uf, vf = ctx.vv_run(dt, nsteps, rng_seed, save_every=1)
+            uf, vf = step(dt, nsteps, rng_seed, save_every=1)
             us.extend(uf[1:]); vs.extend(vf[1:])
             ts.extend(t1 + dt * np.arange(1, nsteps + 1))
         if last:
-            uf, vf = ctx.vv_run(last, 1, rng_seed, save_every=1)
+            uf, vf = step(last, 1, rng_seed, save_every=1)
             us.append(uf[1]); vs.append(vf[1]); ts.append(t2)
     else:
         if nsteps:
-            ctx.vv_run(dt, nsteps, rng_seed)
+            step(dt, nsteps, rng_seed)
         if last:
-            ctx.vv_run(last, 1, rng_seed)
+            step(last, 1, rng_seed)
         u, v = ctx.get_state()
         us.append(u); vs.append(v); ts.append(t2)
     steps = nsteps + (1 if last else 0)

@@ -429,7 +429,7 @@ static void normal_pair(const uint32_t* w, double* n0, double* n1) {
 }
 
 /* RNG streams: c = (atom, step, stream, sub), key = seed */
-enum { RNG_ANDERSEN = 0, RNG_LANGEVIN_A = 1, RNG_LANGEVIN_B = 2 };
+enum { RNG_ANDERSEN = 0, RNG_LANGEVIN_A = 1, RNG_LANGEVIN_B = 2, RNG_EM = 3 };
 
 static void rng_normals3(uint64_t seed, int64_t atom, int64_t step, uint32_t stream,
                          double* out) {
@@ -524,6 +524,45 @@ int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t
         }
     }
     free(a); free(an); free(vh);
+    return nf;
+}
+
+/* Euler-Maruyama for the Langevin SDE, src/nbody_to_ode.jl:567-598.  The drift f is
+ * du = v, dv = sum of the closures - gamma v (:575-589); the noise g is sigma on the
+ * velocity rows only (:592-595).  One EM step from (u, v): u + dt*v (the position rows'
+ * noise term is zero) and v + dt*(a - gamma*v) + sigma*dW, dW = sqrt(dt)*xi. */
+int64_t orc_em_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
+                   uint64_t seed, int64_t save_every, double* u_frames, double* v_frames) {
+    const int64_t n3 = 3 * s->n;
+    const size_t bytes = sizeof(double) * (size_t)(n3 > 0 ? n3 : 1);
+    double* a = (double*)malloc(bytes);
+    const double gamma = s->therm_p;
+    const double sigma = sqrt(2 * gamma * s->kb * s->therm_T / s->ms[0]);
+    const double sqrt_dt = sqrt(dt);
+    int64_t nf = 0;
+    if (save_every > 0 && u_frames) {
+        memcpy(u_frames, u, bytes); memcpy(v_frames, v, bytes); nf = 1;
+    }
+    double xi = 0.0;
+    rhs_impl(s, u, v, a, 0.0, &xi);
+    for (int64_t step = 1; step <= nsteps; ++step) {
+        for (int64_t i = 0; i < s->n; ++i) {
+            double g[3];
+            rng_normals3(seed, i, step, RNG_EM, g);
+            for (int k = 0; k < 3; ++k) {
+                const double vk = v[3 * i + k];
+                u[3 * i + k] = u[3 * i + k] + dt * vk;
+                v[3 * i + k] = vk + dt * (a[3 * i + k] - gamma * vk) + sigma * (sqrt_dt * g[k]);
+            }
+        }
+        rhs_impl(s, u, v, a, 0.0, &xi);
+        if (save_every > 0 && u_frames && step % save_every == 0) {
+            memcpy(u_frames + nf * n3, u, bytes);
+            memcpy(v_frames + nf * n3, v, bytes);
+            ++nf;
+        }
+    }
+    free(a);
     return nf;
 }
 

@@ -98,6 +98,14 @@ void orc_rhs(const orc_system* s, const double* u, const double* v, double* dv);
 int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
                    uint64_t seed, int64_t save_every, double* u_frames, double* v_frames);
 
+/* Euler-Maruyama run of the Langevin SDE of a potential system (src/nbody_to_ode.jl:
+ * 567-598 solved by StochasticDiffEq EM): with a = f(u) (the potential accelerations),
+ * u' = u + dt v and v' = v + dt (a - gamma v) + sigma (sqrt(dt) xi), sigma =
+ * sqrt(2 gamma kb T / m_1); gamma = therm_p, T = therm_T.  xi from the GPU's Philox
+ * stream 3 with counter (atom, step), steps numbered from 1.  Frames as orc_vv_run. */
+int64_t orc_em_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
+                   uint64_t seed, int64_t save_every, double* u_frames, double* v_frames);
+
 /* src/nbody_simulation_result.jl:209-218 */
 double orc_kinetic_energy(const double* v, const double* ms, int64_t n);
 /* src/nbody_simulation_result.jl:293-319 */

@@ -62,6 +62,8 @@ def load():
         "orc_rhs": (None, [sp, _DP, _DP, _DP]),
         "orc_vv_run": (C.c_int64, [sp, _DP, _DP, C.c_double, C.c_int64, C.c_uint64, C.c_int64,
                                    _DP, _DP]),
+        "orc_em_run": (C.c_int64, [sp, _DP, _DP, C.c_double, C.c_int64, C.c_uint64, C.c_int64,
+                                   _DP, _DP]),
         "orc_kinetic_energy": (C.c_double, [_DP, _DP, C.c_int64]),
         "orc_potential_energy": (C.c_double, [sp, _DP]),
         "orc_grav_potential": (C.c_double, [_DP, C.c_int64, _DP, C.c_double]),
@@ -165,6 +167,17 @@ class System:
         uf = np.empty((max(nf, 1),) + u.shape)
         vf = np.empty((max(nf, 1),) + u.shape)
         lib.orc_vv_run(C.byref(self.s), dp(u), dp(v), dt, nsteps, seed, save_every,
+                       dp(uf) if nf else None, dp(vf) if nf else None)
+        return u, v, (uf[:nf], vf[:nf])
+
+    def em_run(self, u, v, dt, nsteps, seed=0, save_every=0):
+        lib = load()
+        u = c3n(u).copy()
+        v = c3n(v).copy()
+        nf = 1 + nsteps // save_every if save_every > 0 else 0
+        uf = np.empty((max(nf, 1),) + u.shape)
+        vf = np.empty((max(nf, 1),) + u.shape)
+        lib.orc_em_run(C.byref(self.s), dp(u), dp(v), dt, nsteps, seed, save_every,
                        dp(uf) if nf else None, dp(vf) if nf else None)
         return u, v, (uf[:nf], vf[:nf])
 

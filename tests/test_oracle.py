@@ -469,3 +469,58 @@ def test_nosehoover_free_particles_follow_the_thermostat_ode(oracle):
     v_eq = v * np.sqrt(T0 * r / Ti)
     _, v1, _ = s.vv_run(u, v_eq, dt, 50)
     assert np.allclose(v1, v_eq, rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Euler-Maruyama for the Langevin SDE (src/nbody_to_ode.jl:567-598)
+# ---------------------------------------------------------------------------
+
+def test_em_free_particles_without_noise_closed_form():
+    """T = 0 (sigma = 0), no forces: v_n = r^n v_0 and x_n = x_0 + v_0 (1 - r^n) / gamma
+    with r = 1 - gamma dt, the exact Euler-Maruyama solution of dv = -gamma v dt."""
+    rng = np.random.default_rng(3)
+    n, gamma, dt, steps = 6, 0.7, 0.01, 300
+    u0, v0 = rng.uniform(-3, 3, (n, 3)), rng.normal(size=(n, 3))
+    s = O.System(np.linspace(1.0, 2.0, n)).thermostat(O.THERM_LANGEVIN, 0.0, gamma)
+    u, v, _ = s.em_run(u0, v0, dt, steps, seed=5)
+    r = 1.0 - gamma * dt
+    assert np.allclose(v, r ** steps * v0, rtol=1e-12, atol=0)
+    assert np.allclose(u, u0 + v0 * (1 - r ** steps) / gamma, rtol=1e-11, atol=1e-13)
+
+
+def test_em_one_step_two_body_gravity():
+    """One step from (x, v) with sigma = 0: x' = x + dt v and v' = v + dt (a(x) - gamma v),
+    with a(x) the two-body Newton acceleration written out here; a dropped force, a
+    force taken at x' or a sign error would not match."""
+    m = np.array([2.0, 5.0])
+    G, gamma, dt = 0.7, 0.3, 0.05
+    u0 = np.array([[0.0, 0.0, 0.0], [1.5, 0.5, -0.2]])
+    v0 = np.array([[0.1, -0.2, 0.3], [-0.4, 0.2, 0.1]])
+    s = O.System(m).add(O.TERM_GRAV, G).thermostat(O.THERM_LANGEVIN, 0.0, gamma)
+    u, v, _ = s.em_run(u0, v0, dt, 1)
+    d = u0[1] - u0[0]
+    r3 = np.linalg.norm(d) ** 3
+    a = np.array([G * m[1] * d / r3, -G * m[0] * d / r3])
+    assert np.allclose(u, u0 + dt * v0, rtol=1e-14, atol=1e-15)
+    assert np.allclose(v, v0 + dt * (a - gamma * v0), rtol=1e-13, atol=1e-15)
+
+
+def test_em_noise_variance_uses_first_body_mass():
+    """Free particles from v = 0: v_{k+1} = r v_k + sigma sqrt(dt) xi, so after N steps
+    every velocity component has variance sigma^2 dt (1 - r^(2N)) / (1 - r^2) with
+    sigma^2 = 2 gamma kb T / m_1 -- the first body's mass for every atom, as in the
+    reference (the light atoms below then sit at about half the target temperature).
+    Tolerance: 5 standard errors of a variance estimated from ~6,000 normal samples."""
+    n, gamma, kT, dt, N = 2001, 1.0, 1.5, 0.01, 400
+    m = np.ones(n)
+    m[0] = 2.0
+    s = O.System(m, kb=1.0).thermostat(O.THERM_LANGEVIN, kT, gamma)
+    u0 = np.zeros((n, 3))
+    _, v, _ = s.em_run(u0, np.zeros((n, 3)), dt, N, seed=11)
+    r = 1.0 - gamma * dt
+    sig2 = 2 * gamma * kT / m[0]
+    want = sig2 * dt * (1 - r ** (2 * N)) / (1 - r * r)
+    x = v[1:].ravel()
+    se = want * math.sqrt(2.0 / x.size)
+    assert abs(x.var() - want) < 5 * se, (x.var(), want)
+    assert abs(x.mean()) < 5 * math.sqrt(want / x.size)
