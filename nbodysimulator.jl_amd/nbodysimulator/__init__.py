@@ -23,5 +23,6 @@ from .types import (AndersenThermostat, BerendsenThermostat, ChargedParticle, Ch
                     PotentialParameters, ReactionFieldParameters, SPCFwParameters,
                     WaterMolecule, WaterSPCFw, get_masses, kb_SI)
 from .generators import generate_bodies_in_cell_nodes
+from .pdb import extract_from_pdb, load_water_molecules_from_pdb, save_to_pdb, write_pdb_data
 
 __all__ = [n for n in dir() if not n.startswith("_")]
