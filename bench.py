@@ -464,6 +464,63 @@ def profile_learned(lib, gen, stride=2, nsteps=200):
     return 0
 
 
+def profile_workloads(lib, gen):
+    """Every benchmark workload briefly, without step graphs (each kernel dispatch is a
+    separate counter sample), for rocprofv3 counter and trace runs: gravity N=65,536
+    (autotuned tile shape, 5 steps), LJ N=32,000 (8 lanes, skin 0.6, 300 steps), water
+    4,096 molecules (autotuned, 100 steps), 512 batched LJ replicas (one 100-step launch),
+    the learned stride (200 collected frames, 50 training iterations, 50 strided steps
+    with a zero network), and a 1 GiB copy kernel (the byte-counter calibration)."""
+    os.environ["NBX_GRAPH_STEPS"] = "0"
+    out = {}
+    n = 65536
+    u, v, m = gen.plummer(n, seed=2024)
+    g = lib.Context(n, lib.SYS_POTENTIAL, m)
+    g.add_gravitational(1.0)
+    g.set_precision(lib.PREC_MIXED)
+    g.set_state(u, v)
+    tuned = g.autotune(1e-3, 20)
+    g.vv_run(1e-3, 5)
+    out["gravity"] = {"targets_per_thread": int(tuned["shape0"]),
+                      "source_chunks": int(tuned["shape1"])}
+    s = gen.lj_liquid(20, rho=0.8442, T=0.9, seed=5)
+    lj = lib.Context(s.u.shape[0], lib.SYS_POTENTIAL, s.masses)
+    lj.set_boundary(lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+    lj.add_lennard_jones(1.0, 1.0, 2.5)
+    lj.set_kb(1.0)
+    lj.set_precision(lib.PREC_MIXED)
+    lj.set_skin(0.6)
+    lj.set_state(s.u, s.v)
+    lj.vv_run(0.005, 300)
+    out["lj32000"] = lj.stats()
+    w = gen.water_box(per_side=16, seed=7)
+    wc = water_context(lib, gen, w, 0.9, True)
+    tuned = wc.autotune(0.0005, 50)
+    wc.vv_run(0.0005, 100)
+    out["water"] = {"lanes": int(tuned["shape0"]), "skin_nm": tuned["shape1"]}
+    reps = gen.lj_batch(512, seed=0)
+    systems = [gen.lj_liquid(r.cells, rho=0.8442, T=r.T, seed=k) for k, r in enumerate(reps)]
+    natoms = [x.u.shape[0] for x in systems]
+    b = lib.Batch(natoms, [x.L for x in systems], np.concatenate([x.masses for x in systems]))
+    for k, r in enumerate(reps):
+        b.set_thermostat(k, lib.THERM_LANGEVIN if r.thermostat == "langevin" else lib.THERM_ANDERSEN,
+                         r.T, 1.0 if r.thermostat == "langevin" else 0.01)
+    b.set_state(np.concatenate([x.u for x in systems]), np.concatenate([x.v for x in systems]))
+    b.run(0.005, 100, steps_per_launch=1000)   # one direct 100-step launch
+    out["batch"] = {"replicas": 512, "atoms": int(sum(natoms))}
+    model = lib.Model(0)
+    lj.learn_collect(model, 0.005, 2, 200, 256, seed=1)
+    model.fit_normalization()
+    model.train(50, batch=8192)
+    zero = lib.Model(0)
+    zero.set(np.zeros(lib.MLP_PARAMS), np.concatenate([np.zeros(35), np.ones(35), np.ones(6)]))
+    lj.learned_run(zero, 0.005, 2, 50, drift_tol=1.0, check_every=25)
+    out["learned"] = {"collected_frames": 200, "train_iters": 50, "strided_steps": 50}
+    out["copy_probe_ms"] = g.copy_probe(1 << 30, 3)   # byte-counter calibration, 1 GiB
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -480,6 +537,9 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the dominant kernels (gravity N=65536 force, 10 launches; "
                          "a 1 GiB copy-kernel calibration, 5 launches) for rocprofv3 counters")
+    ap.add_argument("--profile-workloads", action="store_true",
+                    help="only run every workload briefly without step graphs, for rocprofv3 "
+                         "counter and trace runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
@@ -493,6 +553,8 @@ def main():
         return 1
     if args.profile_kernels:
         return profile_kernels(lib, gen)
+    if args.profile_workloads:
+        return profile_workloads(lib, gen)
     if args.profile_learned:
         return profile_learned(lib, gen)
     log("[bench] gravity N=65536")
