@@ -22,6 +22,7 @@
 #include "nbx_mixed.h"
 
 #define NB_THREADS 256
+#define NB_UNROLL 4
 
 // ---------------------------------------------------------------------------
 // Pair law
@@ -122,15 +123,35 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
         for (int q = 0; q < 6; ++q) acc[q].init();
         const size_t stride = (size_t)n * G;
         const int* p = nbr + gid;
-        for (int k = lane; k < cnt; k += G, p += stride) {
-            const int j = *p;
-            const double4 pj = pos[j];
-            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            dx -= pp.L * rint(dx * pp.invL);
-            dy -= pp.L * rint(dy * pp.invL);
-            dz -= pp.L * rint(dz * pp.invL);
-            const bool ljp = io_ && (!WATER || (j % 3) == 0);
-            nb_pair<LJ, EL>((float)dx, (float)dy, (float)dz, (float)pj.w, ljp, pp, acc);
+        // NB_UNROLL slots per lane and pass: the position gathers of a pass are issued
+        // together, and the slots of the next pass are loaded while this pass computes,
+        // so a lane waits for about one gather per pass rather than two dependent loads
+        // per pair (slots past the count read the atom itself and are skipped)
+        int jn[NB_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NB_UNROLL; ++u) jn[u] = lane + u * G < cnt ? p[u * stride] : i;
+        for (int k = lane; k < cnt; k += NB_UNROLL * G) {
+            int jj[NB_UNROLL];
+            double4 pq[NB_UNROLL];
+#pragma unroll
+            for (int u = 0; u < NB_UNROLL; ++u) jj[u] = jn[u];
+#pragma unroll
+            for (int u = 0; u < NB_UNROLL; ++u) pq[u] = pos[jj[u]];
+            p += NB_UNROLL * stride;
+            const int kn = k + NB_UNROLL * G;
+#pragma unroll
+            for (int u = 0; u < NB_UNROLL; ++u) jn[u] = kn + u * G < cnt ? p[u * stride] : i;
+#pragma unroll
+            for (int u = 0; u < NB_UNROLL; ++u) {
+                if (k + u * G >= cnt) break;
+                const double4 pj = pq[u];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                dx -= pp.L * rint(dx * pp.invL);
+                dy -= pp.L * rint(dy * pp.invL);
+                dz -= pp.L * rint(dz * pp.invL);
+                const bool ljp = io_ && (!WATER || (jj[u] % 3) == 0);
+                nb_pair<LJ, EL>((float)dx, (float)dy, (float)dz, (float)pj.w, ljp, pp, acc);
+            }
         }
         double F[6];
 #pragma unroll
