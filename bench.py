@@ -99,6 +99,19 @@ def mixed_accel(ctx, u):
     return (vf[1] - vf[0]) / dt
 
 
+def gravity_issue_model(ctx, pairs, kernel_ms):
+    """Issue-limited time of k_grav_tiled from the measured issue rates of its
+    instruction kinds (nbx_valu_probe, wave64 instructions per SIMD per ns): per pair
+    3 v_sub + 3 v_mul at the v_mul rate, 6 FMA at the v_fmac rate, one v_rsq_f32."""
+    r_fma, r_mul, r_rsq = (ctx.valu_probe(k, 5) for k in (0, 1, 2))
+    simds = 4 * 256
+    ns_per_wave_pair = 6.0 / r_mul + 6.0 / r_fma + 1.0 / r_rsq     # per 64 pairs, per SIMD
+    bound_ms = pairs / 64.0 * ns_per_wave_pair / simds * 1e-6
+    return {"fmac_per_simd_ns": r_fma, "mul_per_simd_ns": r_mul, "rsq_per_simd_ns": r_rsq,
+            "instructions_per_pair": "3 sub, 3 mul, 6 fma, 1 v_rsq_f32",
+            "issue_bound_ms": bound_ms, "frac_of_issue_bound": bound_ms / kernel_ms}
+
+
 def bench_gravity(args, lib, gen):
     n = 65536
     u, v, m = gen.plummer(n, seed=2024)
@@ -131,6 +144,7 @@ def bench_gravity(args, lib, gen):
         ke2, _, pg2 = ctx.energies()
         drift10k = abs((ke2 + pg2) - e0) / abs(e0)
     achieved = FLOP_GRAV * pairs / (kms * 1e-3) / 1e12
+    issue = gravity_issue_model(ctx, pairs, kms)
     sample_pairs = len(targets) * (n - 1)
     cpu_rate = sample_pairs / cpu_s
     return {
@@ -151,7 +165,8 @@ def bench_gravity(args, lib, gen):
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic_from_profiles(kname),
                      "kernel_avg_ms": kms,
-                     "flop_per_launch": FLOP_GRAV * pairs},
+                     "flop_per_launch": FLOP_GRAV * pairs,
+                     "issue_model": issue},
         "cpu_baseline": {"value": cpu_rate, "unit": "pair interactions/s", "cores": 1,
                          "kind": "port",
                          "sample": f"FP64 oracle (reference per-atom loop restated in C, "

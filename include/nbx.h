@@ -188,6 +188,11 @@ int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);
  * launches on the context's stream.  Calibrates HBM byte counters (the kernel reads
  * and writes exactly `bytes` each) and gives the achievable bandwidth. */
 int nbx_copy_probe(nbx_ctx* ctx, int64_t bytes, int32_t iters, double* avg_ms);
+/* Measurement support (no reference counterpart): VALU issue rate, in wave64
+ * instructions per SIMD per nanosecond (best of `reps` launches), of a stream of
+ * independent instructions of one kind: 0 v_fmac_f32, 1 v_mul_f32, 2 v_rsq_f32.  With
+ * the instruction counts of a kernel it gives the issue-limited time of that kernel. */
+int nbx_valu_probe(nbx_ctx* ctx, int32_t kind, int32_t reps, double* rate);
 /* Autotuner (no reference counterpart): times `trial_steps` mixed-path steps of length dt
  * from the current state for every candidate launch shape and keeps the fastest -- list
  * plans: lanes per atom {2, 4, 8, 16} x Verlet skin {0.06 .. 0.40} x cutoff; gravity:

@@ -797,6 +797,13 @@ extern "C" int nbx_copy_probe(nbx_ctx* c, int64_t bytes, int32_t iters, double* 
     return mx_copy_probe(c, bytes, iters, avg_ms);
 }
 
+extern "C" int nbx_valu_probe(nbx_ctx* c, int32_t kind, int32_t reps, double* rate) {
+    if (!c || !rate) return NBX_ERR_INVALID;
+    if (kind < 0 || kind > 2 || reps <= 0)
+        return nbx_fail(c, NBX_ERR_INVALID, "kind 0..2 and reps > 0 required");
+    return mx_valu_probe(c, kind, reps, rate);
+}
+
 static int learn_ready(nbx_ctx* c) {
     if (!c->state_set) return nbx_fail(c, NBX_ERR_INVALID, "no state set");
     if (c->prec != NBX_PREC_MIXED)

@@ -222,6 +222,66 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ s
         dst[i] = src[i];
 }
 
+// VALU issue-rate calibration: 8 independent chains per lane of one instruction, its
+// encoding pinned by inline asm (0: v_fmac_f32, 1: v_mul_f32, 2: v_rsq_f32)
+#define VP_ITERS 2048
+template <int KIND>
+__global__ void __launch_bounds__(256) k_valu_probe(float* __restrict__ out, float b0, float c0) {
+    float a[8];
+    const float b = b0 + threadIdx.x * 1e-9f, cc = c0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = 1.0f + threadIdx.x * 1e-3f + k;
+    for (int it = 0; it < VP_ITERS; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (KIND == 0) asm volatile("v_fmac_f32 %0, %1, %2" : "+v"(a[k]) : "v"(b), "v"(cc));
+            if (KIND == 1) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(a[k]) : "v"(b));
+            if (KIND == 2) asm volatile("v_rsq_f32 %0, %0" : "+v"(a[k]));
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += a[k];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+int mx_valu_probe(nbx_ctx* c, int kind, int reps, double* rate) {
+    const int blocks = 8192, threads = 256;
+    float* out = nullptr;
+    if (hipMalloc((void**)&out, sizeof(float) * blocks * threads) != hipSuccess)
+        return nbx_fail(c, NBX_ERR_DEVICE, "valu probe: out of memory");
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    hipStream_t s = c->stream;
+    auto launch = [&]() {
+        if (kind == 0) hipLaunchKernelGGL(k_valu_probe<0>, dim3(blocks), dim3(threads), 0, s, out, 0.999f, 1e-3f);
+        else if (kind == 1) hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(threads), 0, s, out, 0.999f, 1e-3f);
+        else hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(threads), 0, s, out, 0.999f, 1e-3f);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 0.f;
+    hipError_t err = hipSuccess;
+    for (int r = 0; r < reps && err == hipSuccess; ++r) {
+        (void)hipEventRecord(e0, s);
+        launch();
+        (void)hipEventRecord(e1, s);
+        err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r == 0 || ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (err != hipSuccess) return nbx_fail(c, NBX_ERR_DEVICE, hipGetErrorString(err));
+    const double wave_instr = (double)blocks * threads / 64 * VP_ITERS * 8;
+    *rate = wave_instr / (4.0 * (cus > 0 ? cus : 256)) / ((double)best * 1e6);
+    return NBX_OK;
+}
+
 int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms) {
     const int64_t n16 = bytes / 16;
     void* a = nullptr;

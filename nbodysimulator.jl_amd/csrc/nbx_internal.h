@@ -202,6 +202,7 @@ int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d
 
 // ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name);
+int mx_valu_probe(nbx_ctx* c, int kind, int reps, double* rate);
 int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms);
 int mx_autotune(nbx_ctx* c, double dt, int trial_steps, double* out, int nout);
 // learned stride (nbx_mixed.cpp)

@@ -66,6 +66,7 @@ SIGNATURES = {
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
     "nbx_copy_probe": (C.c_int, [_P, _I64, _I32, _DP]),
+    "nbx_valu_probe": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
     "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
     "nbx_autotune": (C.c_int, [_P, _D, _I32, _DP, _I32]),
     "nbx_batch_create": (C.c_int, [C.POINTER(_P), _I32, C.POINTER(_I64), _DP, _DP, _D, _D,
@@ -317,6 +318,13 @@ class Context:
         p = C.c_int64()
         self._check(self._lib.nbx_count_pairs(self._h, C.byref(p)))
         return p.value
+
+    def valu_probe(self, kind: int, reps: int = 3) -> float:
+        """Wave64 VALU instructions per SIMD per ns of an independent stream of one
+        instruction kind (0 v_fmac_f32, 1 v_mul_f32, 2 v_rsq_f32)."""
+        r = C.c_double()
+        self._check(self._lib.nbx_valu_probe(self._h, int(kind), int(reps), C.byref(r)))
+        return r.value
 
     def copy_probe(self, nbytes: int, iters: int = 10) -> float:
         """Average duration (ms) of a plain copy kernel moving `nbytes` (read + write)."""

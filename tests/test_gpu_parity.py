@@ -305,6 +305,8 @@ def test_mixed_count_pairs_and_kernel_timer(c1):
     assert pairs == int((r2 < 6.25).sum())
     ms, name = ctx.time_force_kernel(5)
     assert ms > 0 and name == "k_nb_list"
+    # issue-rate calibration: an independent v_rsq_f32 stream issues slower than v_fmac_f32
+    assert ctx.valu_probe(0, 2) > ctx.valu_probe(2, 2) > 0
 
 
 # ---------------------------------------------------------------------------
