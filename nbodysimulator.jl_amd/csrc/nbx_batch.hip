@@ -1,7 +1,8 @@
 // nbx_batch.hip -- batched Lennard-Jones replicas in one persistent launch (gfx950).
 //
 // One workgroup of BATCH_THREADS threads owns one replica for every step of a launch:
-//   - the replica's positions, wrapped into [0, L) and rounded to FP32, live in LDS
+//   - the replica's positions, wrapped into [0, L), centred on the box and rounded to
+//     FP32, live in LDS
 //     (BATCH_NMAX float4 = 64 KB), so the pair loop never reads HBM for positions;
 //   - x, v, a, vh and the list reference positions stay in FP64 in HBM (SoA over the
 //     whole batch: component k of global atom g at k * ntot + g); a thread owns atoms
@@ -50,20 +51,24 @@ struct nbx_batch_params {
     int64_t ntot;
 };
 
+// position wrapped into [0, L) in FP64, then centred on the box and rounded to FP32:
+// coordinates in [-L/2, L/2) carry half the rounding error of [0, L)
 __device__ __forceinline__ float4 batch_wrap(double x, double y, double z, double L) {
     double wx = x - L * floor(x / L), wy = y - L * floor(y / L), wz = z - L * floor(z / L);
     if (wx >= L) wx -= L;
     if (wy >= L) wy -= L;
     if (wz >= L) wz -= L;
-    return make_float4((float)wx, (float)wy, (float)wz, 0.0f);
+    const double h = 0.5 * L;
+    return make_float4((float)(wx - h), (float)(wy - h), (float)(wz - h), 0.0f);
 }
 
-// cell of a wrapped position (nc cells per side, inv_cs = nc / L)
-__device__ __forceinline__ int batch_cell(float4 w, double inv_cs, int nc) {
-    int cx = (int)(w.x * inv_cs), cy = (int)(w.y * inv_cs), cz = (int)(w.z * inv_cs);
-    cx = cx < nc ? cx : nc - 1;
-    cy = cy < nc ? cy : nc - 1;
-    cz = cz < nc ? cz : nc - 1;
+// cell of a centred position (nc cells per side, inv_cs = nc / L, h = L / 2)
+__device__ __forceinline__ int batch_cell(float4 w, double inv_cs, double h, int nc) {
+    int cx = (int)((w.x + h) * inv_cs), cy = (int)((w.y + h) * inv_cs),
+        cz = (int)((w.z + h) * inv_cs);
+    cx = cx < 0 ? 0 : (cx < nc ? cx : nc - 1);
+    cy = cy < 0 ? 0 : (cy < nc ? cy : nc - 1);
+    cz = cz < 0 ? 0 : (cz < nc ? cz : nc - 1);
     return (cz * nc + cy) * nc + cx;
 }
 
@@ -154,7 +159,8 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 if (i < n) {
                     const double x0 = X0[i], x1 = X1[i], x2 = X2[i];
                     XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
-                    const int c = batch_cell(batch_wrap(x0, x1, x2, rep.L), rep.inv_cs, nc);
+                    const int c = batch_cell(batch_wrap(x0, x1, x2, rep.L), rep.inv_cs,
+                                             0.5 * rep.L, nc);
                     code[m] = c * BATCH_NMAX + atomicAdd(&ccount[c], 1);
                 }
             }
@@ -207,7 +213,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             for (int s = tid; s < n; s += BATCH_THREADS) {
                 const int i = tidx[s];
                 const float4 pi = spos[s];
-                const int c0 = batch_cell(pi, rep.inv_cs, nc);
+                const int c0 = batch_cell(pi, rep.inv_cs, 0.5 * rep.L, nc);
                 const int cz = c0 / (nc * nc), cy = (c0 / nc) % nc, cx = c0 % nc;
                 int cnt = 0;
                 uint64_t lo = 0, hi = 0;

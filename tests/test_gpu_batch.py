@@ -75,8 +75,13 @@ def test_batch_thermostats_reach_targets(nbx):
     U = np.concatenate([s.u for s in systems])
     V = np.concatenate([s.v for s in systems])
     b.set_state(U, V)
-    b.run(0.005, 3000, seed=11)
-    ke, _, _ = b.energies()
+    b.run(0.005, 2000, seed=11)
+    # kinetic temperature averaged over 10 frames 100 steps apart (one frame of a
+    # 108..500-atom replica fluctuates by ~5 %)
+    ke = np.zeros(len(systems))
+    for _ in range(10):
+        b.run(0.005, 100, seed=11)
+        ke += b.energies()[0] / 10
     for r, (s, t) in enumerate(zip(systems, targets)):
         n = s.u.shape[0]
         T = 2 * ke[r] / (3 * n)
