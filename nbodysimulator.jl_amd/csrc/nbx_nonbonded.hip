@@ -441,7 +441,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
               const int* __restrict__ atoms, int ccap, float rlist2f, int G, int cap,
               int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
               int* __restrict__ flags, const int* __restrict__ req,
-              int* __restrict__ clear_count) {
+              int* __restrict__ clear_count, int parts) {
     __shared__ int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
     __shared__ int sord[BUILD_SLOTS];
@@ -453,10 +453,12 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int cid = blockIdx.x;
-    if (clear_count != nullptr && tid == 0) clear_count[cid] = 0;
+    // `parts` blocks share a home cell, each taking a contiguous slice of its atoms (more
+    // blocks in flight when the cells are few and large, as for water)
+    const int cid = blockIdx.x / parts, part = blockIdx.x % parts;
+    if (clear_count != nullptr && tid == 0 && part == 0) clear_count[cid] = 0;
     if (req[0] == 0) return;
-    if (cid == 0 && tid == 0) atomicAdd(&flags[2], 1);
+    if (blockIdx.x == 0 && tid == 0) atomicAdd(&flags[2], 1);
     const int cz = cid / (nc * nc), cy = (cid / nc) % nc, cx = cid % nc;
     if (tid < 27) {  // the 27 neighbour cells (fixed order) and their counts, in parallel
         const int oz = tid / 9 - 1, oy = (tid / 3) % 3 - 1, ox = tid % 3 - 1;
@@ -467,11 +469,14 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     }
     int m = count[cid];
     m = m < ccap ? m : ccap;
+    const int per = (m + parts - 1) / parts;
+    const int hb = part * per, he = (hb + per) < m ? (hb + per) : m;
+    if (hb >= he) return;   // uniform over the block, before its first barrier
     __syncthreads();
     const size_t stride = (size_t)n * G;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int own0 = 0; own0 < m; own0 += BUILD_OWN) {
-        const int mo = (m - own0) < BUILD_OWN ? (m - own0) : BUILD_OWN;
+    for (int own0 = hb; own0 < he; own0 += BUILD_OWN) {
+        const int mo = (he - own0) < BUILD_OWN ? (he - own0) : BUILD_OWN;
         for (int t = tid; t < mo; t += BUILD_THREADS) {
             const int i = atoms[(size_t)cid * ccap + own0 + t];
             const double4 p = pos[i];
@@ -599,16 +604,20 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
     if (nc >= 3) {
         const int ncells = nc * nc * nc;
         const float r2f = (float)r2 * 1.0001f;
+        // at least ~512 blocks (two per CU) when the cells are few: up to 8 per cell
+        int parts = (512 + ncells - 1) / ncells;
+        parts = parts < 1 ? 1 : (parts > 8 ? 8 : parts);
+        const dim3 grid(ncells * parts);
         if (water)
-            hipLaunchKernelGGL(k_build_cells<true>, dim3(ncells), dim3(BUILD_THREADS), 0, s,
+            hipLaunchKernelGGL(k_build_cells<true>, grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
                                bin_count, bin_atoms, nl.ccap, r2f, nl.lanes, nl.cap, nl.d_nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count);
+                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, parts);
         else
-            hipLaunchKernelGGL(k_build_cells<false>, dim3(ncells), dim3(BUILD_THREADS), 0, s,
+            hipLaunchKernelGGL(k_build_cells<false>, grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
                                bin_count, bin_atoms, nl.ccap, r2f, nl.lanes, nl.cap, nl.d_nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count);
+                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, parts);
     } else {
         if (water)
             hipLaunchKernelGGL(k_build_allpairs<true>, dim3(nbx_blocks(n, 128)), dim3(128), 0, s,
