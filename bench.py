@@ -399,6 +399,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
 
     def run(model, stride):
         ctx, _ = ctx_for(4242)
+        ctx.autotune(dt, 200)     # the launch shape of the plain reference run below
         t0 = time.perf_counter()
         rep = ctx.learned_run(model, dt, stride, nsteps_dt // stride, drift_tol=1e-4,
                               check_every=25)
