@@ -218,8 +218,19 @@ int* bin_atoms(nbx_ctx* c, int par) {
     return c->nl.d_cell_atoms ? c->nl.d_cell_atoms + par * ncells * (size_t)c->nl.ccap : nullptr;
 }
 
+mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
+    mx_nh_params nh{};
+    nh.nh = c->therm == NBX_THERM_NOSEHOOVER ? 1 : 0;
+    nh.advance = advance;
+    nh.dt = dt;
+    nh.T0 = c->therm_T;
+    nh.tau = c->therm_p;
+    nh.ndf = 3 * c->n - (c->sys == NBX_SYS_WATER ? 2 * (c->n / 3) : 0);
+    return nh;
+}
+
 // buffers of a launch that reads x in X[par] and (when drifting) writes X[par ^ 1]
-mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift) {
+mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0.0) {
     mx_io io{};
     io.x_in = xbuf(c, par);
     io.x_out = xbuf(c, par ^ 1);
@@ -245,45 +256,26 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift) {
     io.red_counter = reinterpret_cast<unsigned*>(c->nl.d_flags + 4);
     io.T_out = c->d_scal;
     io.T_denom = c->kb * (double)(3 * c->n - constraints(c));
+    io.nh = nh_params(c, dt, 1);   // with need_T: the Nose-Hoover update of this drift
     return io;
 }
 
-mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
-    mx_nh_params nh{};
-    nh.nh = c->therm == NBX_THERM_NOSEHOOVER ? 1 : 0;
-    nh.advance = advance;
-    nh.dt = dt;
-    nh.T0 = c->therm_T;
-    nh.tau = c->therm_p;
-    nh.ndf = 3 * c->n - (c->sys == NBX_SYS_WATER ? 2 * (c->n / 3) : 0);
-    return nh;
-}
 
-// after a kernel that wrote x into X[par_out]: bins for a requested list build, and the
-// thermostat temperature of vh from the kernel's `nparts` per-block partials (Berendsen;
-// Nose-Hoover also advances zeta by dt and sets its rate)
-void after_drift(nbx_ctx* c, const mx_plan& p, int par_out, int nparts, double dt) {
+// after a kernel that wrote x into X[par_out]: bins for a requested list build (the
+// thermostat temperature of vh, and the Nose-Hoover update, are finished by the last
+// block of that kernel: mx_epilogue)
+void after_drift(nbx_ctx* c, const mx_plan& p, int par_out) {
     hipStream_t s = c->stream;
     if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3)
         mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
                       bin_atoms(c, par_out));
-    if (c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER)
-        mx_launch_T_final(s, c->d_red, nparts, c->kb * (double)(3 * c->n - constraints(c)),
-                          c->d_scal, nh_params(c, dt, 1));
-}
-
-// blocks of the kernel that ends a step (its per-block temperature partials)
-int step_blocks(const nbx_ctx* c, const mx_plan& p) {
-    if (p.kind == PLAN_WATER) return (int)nbx_blocks(c->n / 3, 128);
-    if (p.kind == PLAN_NONBONDED && p.nb.mode_list) return (int)nbx_blocks(c->n * c->nl.lanes, 256);
-    return (int)nbx_blocks(c->n, 256);
 }
 
 // force evaluation at x in X[par] (list: P[par]); with kp.do_kick the step's end, with
 // do_drift the start of the next step
 void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par, int do_drift) {
     hipStream_t s = c->stream;
-    const mx_io io = make_io(c, p, par, do_drift);
+    const mx_io io = make_io(c, p, par, do_drift, kp.dt);
     if (p.kind == PLAN_GRAVITY) {
         mx_launch_gravity(s, c, pbuf(c, par), p.G, kp, io, 0);
     } else if (p.kind == PLAN_MAGNETIC) {
@@ -295,7 +287,7 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
         mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io, clear);
         if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
     }
-    if (do_drift) after_drift(c, p, par ^ 1, step_blocks(c, p), kp.dt);
+    if (do_drift) after_drift(c, p, par ^ 1);
 }
 
 void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
@@ -499,8 +491,8 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 mx_kick_params kp = kick_params(c, 1, dt, seed);
                 kp.step_base = c->d_step;
                 kp.step_off = 0;
-                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1));
-                after_drift(c, p, 1, (int)nbx_blocks(n, 256), dt);
+                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1, dt));
+                after_drift(c, p, 1);
             }
             const int gs = c->graph_steps;
             int64_t k0 = 0;
@@ -882,8 +874,8 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         ip.dlab = dlab.as<float>();
         ip.part = mpart.as<double>();
         lr_launch_mlp_infer(s, ip);
-        lr_launch_drift(s, mpart.as<double>(), nblk, dlab.as<float>(), D, make_io(c, p, par, 1), n);
-        after_drift(c, p, par ^ 1, 0, D);
+        lr_launch_drift(s, mpart.as<double>(), nblk, dlab.as<float>(), D, make_io(c, p, par, 1, D), n);
+        after_drift(c, p, par ^ 1);
     };
     auto run_chunk = [&](int64_t msteps) -> int {
         int rc2;

@@ -45,6 +45,14 @@ struct mx_kick_params {
     double em_gamma, em_sigma, em_sqrt_dt;
 };
 
+// Nose-Hoover update at the end of a temperature reduction (src/thermostats.jl:116-128):
+// zeta += dt xi (advance), then xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
+struct mx_nh_params {
+    int nh, advance;
+    double dt, T0, tau;
+    int64_t ndf;
+};
+
 // Buffers of one fused step launch
 struct mx_io {
     const double* x_in;   // x_k (SoA, unwrapped)
@@ -72,6 +80,7 @@ struct mx_io {
     unsigned* red_counter;
     double* T_out;
     double T_denom;       // kb (3N - Nc)
+    mx_nh_params nh;      // Nose-Hoover update applied with the temperature
 };
 
 // resolve the absolute step number inside a kernel (captured graphs are replayed with a
@@ -206,21 +215,51 @@ __device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
 }
 
 // Epilogue of a step kernel, called by every thread of every block (blockDim.x <= 256):
-// with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x]; the
-// finalize kernel launched next adds the partials in block order (deterministic).  No
-// cross-block fence is needed: the kernel boundary orders the partials before the sum.
+// with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x]; the last
+// block to finish (device-scope counter) adds the partials in block order, writes the
+// temperature of vh_{k+1} and applies the Nose-Hoover update.  Deterministic: the final
+// sum does not depend on which block finishes last.
 __device__ __forceinline__ void mx_epilogue(double val, const mx_io& io, int n) {
     (void)n;
     if (!io.need_T) return;
     __shared__ double red[256];
+    __shared__ int last;
     const int t = threadIdx.x;
+    const int nt = (int)blockDim.x;
     red[t] = val;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (t < off && t + off < (int)blockDim.x) red[t] += red[t + off];
+        if (t < off && t + off < nt) red[t] += red[t + off];
         __syncthreads();
     }
-    if (t == 0) io.red_part[blockIdx.x] = red[0];
+    if (t == 0) {
+        io.red_part[blockIdx.x] = red[0];
+        __threadfence();
+        last = atomicAdd(io.red_counter, 1u) == gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int b = t; b < (int)gridDim.x; b += nt) s += __builtin_nontemporal_load(io.red_part + b);
+    red[t] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off && t + off < nt) red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double T = red[0] / io.T_denom;
+        io.T_out[0] = T;
+        if (io.nh.nh) {
+            if (io.nh.advance)
+                io.T_out[NBX_SCAL_ZETA] = io.T_out[NBX_SCAL_ZETA] + io.nh.dt * io.T_out[NBX_SCAL_XI];
+            const double it = 1.0 / io.nh.tau;
+            io.T_out[NBX_SCAL_XI] =
+                (it * it) * (T / io.nh.T0 - (double)(io.nh.ndf + 1) / (double)io.nh.ndf);
+        }
+        *io.red_counter = 0u;   // ready for the next launch
+    }
 }
 
 // launchers (nbx_nonbonded.hip / nbx_gravity.hip)
@@ -265,11 +304,4 @@ void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* 
 // T = (sum of nparts per-block partials, in order) / denom into scal[0]; with a
 // Nose-Hoover thermostat (nh != 0) also zeta += dt xi (advance != 0) and then
 // xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
-struct mx_nh_params {
-    int nh, advance;
-    double dt, T0, tau;
-    int64_t ndf;
-};
-void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* scal,
-                       const mx_nh_params& nh);
 void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh);

@@ -365,36 +365,6 @@ void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* 
                        c->nl.d_flags, req);
 }
 
-// temperature from the per-block partials of the preceding step kernel (block order)
-__global__ void __launch_bounds__(256)
-k_T_final(const double* __restrict__ part, int nparts, double denom, double* __restrict__ scal,
-          mx_nh_params nh) {
-    __shared__ double red[256];
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int b = t; b < nparts; b += 256) s += part[b];
-    red[t] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off) red[t] += red[t + off];
-        __syncthreads();
-    }
-    if (t == 0) {
-        const double T = red[0] / denom;
-        scal[0] = T;
-        if (nh.nh) {
-            if (nh.advance) scal[NBX_SCAL_ZETA] = scal[NBX_SCAL_ZETA] + nh.dt * scal[NBX_SCAL_XI];
-            const double it = 1.0 / nh.tau;
-            scal[NBX_SCAL_XI] = (it * it) * (T / nh.T0 - (double)(nh.ndf + 1) / (double)nh.ndf);
-        }
-    }
-}
-
-void mx_launch_T_final(hipStream_t s, const double* part, int nparts, double denom, double* scal,
-                       const mx_nh_params& nh) {
-    hipLaunchKernelGGL(k_T_final, dim3(1), dim3(256), 0, s, part, nparts, denom, scal, nh);
-}
-
 // xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf) from scal[0] (the initial right-hand side)
 __global__ void k_nh_rate(double* scal, mx_nh_params nh) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
