@@ -616,34 +616,45 @@ __device__ __forceinline__ void mol_bond(const double* ri, const double* rj, dou
     f[2] += fac * rz;
 }
 
-__global__ void __launch_bounds__(128)
-k_water_molecules(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_params kp,
-                  mx_io io) {
+// One thread per atom, 64 molecules (192 atoms) per block: the block's positions are
+// shared through LDS, every thread evaluates its molecule's bonds and angle (the same
+// expressions in the same order for all three) and keeps its own atom's terms, then ends
+// its atom's step.  Andersen draws one uniform per molecule (counter of its O atom).
+#define WATER_MOLS 64
+__global__ void __launch_bounds__(3 * WATER_MOLS)
+k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_params kp,
+              mx_io io) {
     mx_resolve_step(kp);
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sx[3][3 * WATER_MOLS];
+    const int t = threadIdx.x;
+    const int i = 3 * WATER_MOLS * blockIdx.x + t;
+    const bool act = i < n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sx[k][t] = act ? io.x_in[(size_t)k * n + i] : 0.0;
+    __syncthreads();
     double tval = 0.0;
-    if (m < n / 3) {
-        const double* __restrict__ x = io.x_in;
-        const int o = 3 * m;
+    if (act) {
+        const int w = t % 3;               // 0: O, 1: H1, 2: H2
+        const int b = t - w;               // the molecule's O in this block
+        const int o = i - w;
         double r[3][3];
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            r[t][0] = x[o + t];
-            r[t][1] = x[n + o + t];
-            r[t][2] = x[2 * n + o + t];
-        }
-        double f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        mol_bond(r[0], r[1], rOH, kbond, f[0]);
-        mol_bond(r[0], r[2], rOH, kbond, f[0]);
-        mol_bond(r[1], r[0], rOH, kbond, f[1]);
-        mol_bond(r[2], r[0], rOH, kbond, f[2]);
-        double acc[3][3];
+        for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const double im = 1.0 / io.ms[o + t];
-            for (int k = 0; k < 3; ++k) acc[t][k] = io.a[k * n + o + t] + im * f[t][k];
+            for (int k = 0; k < 3; ++k) r[q][k] = sx[k][b + q];
+        double f[3] = {0, 0, 0};
+        if (w == 0) {
+            mol_bond(r[0], r[1], rOH, kbond, f);
+            mol_bond(r[0], r[2], rOH, kbond, f);
+        } else {
+            mol_bond(r[w], r[0], rOH, kbond, f);
         }
-        // angle H1-O-H2
+        const double mi = io.ms[i];
+        const double im = 1.0 / mi;
+        double acc[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] = io.a[(size_t)k * n + i] + im * f[k];
+        // angle H1-O-H2 (src/basic_potentials.jl:395-433)
         double rba[3], rbc[3], rcb[3];
         for (int k = 0; k < 3; ++k) {
             rba[k] = r[1][k] - r[0][k];
@@ -666,32 +677,25 @@ k_water_molecules(int n, double rOH, double kbond, double aHOH, double ka, mx_ki
         for (int k = 0; k < 3; ++k) {
             const double fa = ia * ta[k] * force / nba;
             const double fc = ic * tc[k] * force / nbc;
-            acc[1][k] += fa / io.ms[o + 1];
-            acc[0][k] += -(fa + fc) / io.ms[o];
-            acc[2][k] += fc / io.ms[o + 2];
+            if (w == 1) acc[k] += fa / mi;
+            else if (w == 0) acc[k] += -(fa + fc) / mi;
+            else acc[k] += fc / mi;
         }
         if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
-            for (int t = 0; t < 3; ++t)
-                for (int k = 0; k < 3; ++k) acc[t][k] += cb * io.vh[k * n + o + t];
+            for (int k = 0; k < 3; ++k) acc[k] += cb * io.vh[(size_t)k * n + i];
         }
-        for (int t = 0; t < 3; ++t)
-            for (int k = 0; k < 3; ++k) io.a[k * n + o + t] = acc[t][k];
+        for (int k = 0; k < 3; ++k) io.a[(size_t)k * n + i] = acc[k];
         if (kp.do_kick) {
-            double v3[3][3];
-            for (int t = 0; t < 3; ++t)
-                mx_kick_atom(o + t, n, acc[t][0], acc[t][1], acc[t][2], io.vh, io.ms, kp, v3[t]);
+            double v3[3];
+            mx_kick_atom(i, n, acc[0], acc[1], acc[2], io.vh, io.ms, kp, v3);
             if (kp.therm == NBX_THERM_ANDERSEN &&
                 nbx_rng_uniform(kp.seed, o, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
-                for (int t = 0; t < 3; ++t) mx_andersen_atom(o + t, io.ms[o + t], kp, v3[t]);
-            for (int t = 0; t < 3; ++t) {
-                io.v[o + t] = v3[t][0];
-                io.v[n + o + t] = v3[t][1];
-                io.v[2 * n + o + t] = v3[t][2];
-            }
-            if (io.do_drift)
-                for (int t = 0; t < 3; ++t)
-                    tval += mx_drift_atom(o + t, n, v3[t], acc[t][0], acc[t][1], acc[t][2], kp, io);
+                mx_andersen_atom(i, mi, kp, v3);
+            io.v[i] = v3[0];
+            io.v[n + i] = v3[1];
+            io.v[2 * n + i] = v3[2];
+            if (io.do_drift) tval = mx_drift_atom(i, n, v3, acc[0], acc[1], acc[2], kp, io);
         }
     }
     mx_epilogue(tval, io, n);
@@ -701,8 +705,8 @@ void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_pa
                                const mx_io& io) {
     const int64_t nmol = c->n / 3;
     if (nmol <= 0) return;
-    hipLaunchKernelGGL(k_water_molecules, dim3(nbx_blocks(nmol, 128)), dim3(128), 0, s,
-                       (int)c->n, c->rOH, c->kbond, c->aHOH, c->kangle, kp, io);
+    hipLaunchKernelGGL(k_water_atoms, dim3(nbx_blocks(nmol, WATER_MOLS)), dim3(3 * WATER_MOLS),
+                       0, s, (int)c->n, c->rOH, c->kbond, c->aHOH, c->kangle, kp, io);
 }
 
 // ---------------------------------------------------------------------------
