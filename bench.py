@@ -316,6 +316,7 @@ def bench_batch(args, lib, gen):
         else:
             b.set_thermostat(k, lib.THERM_ANDERSEN, r.T, 0.01)
     b.set_state(np.concatenate([s.u for s in systems]), np.concatenate([s.v for s in systems]))
+    tuned = b.autotune(0.005, 200)
     steps = 5000
     b.run(0.005, 10)
     t0 = time.perf_counter()
@@ -334,6 +335,7 @@ def bench_batch(args, lib, gen):
     n2 = np.asarray(natoms, dtype=np.float64) ** 2
     cpu_step_s = cpu_s * n2.sum() / n2[pick].sum()
     return {"replicas": 512, "atoms": int(sum(natoms)), "steps": steps,
+            "autotuned": {"skin": tuned["skin"], "ms_per_step_default": tuned["entry_ms_per_step"]},
             "ms_per_step": 1e3 * el / steps, "atom_steps_per_s": sum(natoms) * steps / el,
             "pairs_per_step": int(pairs.sum()),
             "pair_interactions_per_s": float(pairs.sum()) * steps / el,

@@ -229,6 +229,11 @@ const char* nbx_batch_last_error(const nbx_batch* b);
 int nbx_batch_set_thermostat(nbx_batch* b, int32_t replica, int32_t kind, double T,
                              double param);
 int nbx_batch_set_skin(nbx_batch* b, double skin);
+/* Autotuner (no reference counterpart): times `trial_steps` steps of length dt from the
+ * current state for the current skin and for skins {0.2, 0.3, 0.4, 0.5, 0.6, 0.8} x sigma,
+ * restoring the state after each trial, and keeps the fastest.  out[3] (or NULL): the
+ * chosen skin, its ms per step, the ms per step of the skin on entry. */
+int nbx_batch_autotune(nbx_batch* b, double dt, int64_t trial_steps, double* out);
 /* u, v: 3 x sum(natoms) column-major, replicas concatenated */
 int nbx_batch_set_state(nbx_batch* b, const double* u, const double* v);
 int nbx_batch_get_state(const nbx_batch* b, double* u, double* v);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <numeric>
 #include <string>
@@ -378,6 +379,56 @@ extern "C" int nbx_batch_run(nbx_batch* b, double dt, int64_t nsteps, uint64_t s
     }
     b->accel_valid = true;
     b->steps_taken = step0 + nsteps;
+    return NBX_OK;
+}
+
+// Verlet skin by timing: for every candidate (in units of sigma) `trial_steps` steps of
+// length dt from the current state, which is restored after each trial (positions,
+// velocities, accelerations, step count).  Keeps the fastest skin.
+extern "C" int nbx_batch_autotune(nbx_batch* b, double dt, int64_t trial_steps, double* out) {
+    if (!b) return NBX_ERR_INVALID;
+    if (!b->state_set) return bfail(b, NBX_ERR_INVALID, "no state set");
+    if (!(dt > 0) || !std::isfinite(dt) || trial_steps <= 0)
+        return bfail(b, NBX_ERR_INVALID, "dt > 0 and trial_steps > 0 required");
+    static const double cand[] = {0.2, 0.3, 0.4, 0.5, 0.6, 0.8};
+    hipStream_t s = b->stream;
+    const size_t b3 = sizeof(double) * 3 * (size_t)b->ntot;
+    double* bk = nullptr;
+    BHIP(b, hipMalloc((void**)&bk, 3 * b3));
+    struct guard { double* p; ~guard() { if (p) (void)hipFree(p); } } g{bk};
+    BHIP(b, hipMemcpyAsync(bk, b->d_x, b3, hipMemcpyDeviceToDevice, s));
+    BHIP(b, hipMemcpyAsync((char*)bk + b3, b->d_v, b3, hipMemcpyDeviceToDevice, s));
+    BHIP(b, hipMemcpyAsync((char*)bk + 2 * b3, b->d_a, b3, hipMemcpyDeviceToDevice, s));
+    const double skin0 = b->skin;
+    const int64_t steps0 = b->steps_taken;
+    const bool accel0 = b->accel_valid;
+    double best_ms = 0, best_skin = skin0, entry_ms = -1;
+    auto restore = [&]() -> int {
+        BHIP(b, hipMemcpyAsync(b->d_x, bk, b3, hipMemcpyDeviceToDevice, s));
+        BHIP(b, hipMemcpyAsync(b->d_v, (char*)bk + b3, b3, hipMemcpyDeviceToDevice, s));
+        BHIP(b, hipMemcpyAsync(b->d_a, (char*)bk + 2 * b3, b3, hipMemcpyDeviceToDevice, s));
+        BHIP(b, hipStreamSynchronize(s));
+        b->steps_taken = steps0;
+        return NBX_OK;
+    };
+    for (int k = -1; k < (int)(sizeof(cand) / sizeof(cand[0])); ++k) {
+        const double skin = k < 0 ? skin0 : cand[k] * b->sigma;
+        int rc = nbx_batch_set_skin(b, skin);
+        if (rc) return rc;
+        if ((rc = nbx_batch_run(b, dt, 20, 0, 0))) return rc;           // warm: lists, graph
+        auto t0 = std::chrono::steady_clock::now();
+        if ((rc = nbx_batch_run(b, dt, trial_steps, 0, 0))) return rc;  // synchronous
+        const double ms = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0).count() / trial_steps;
+        if ((rc = restore())) return rc;
+        if (k < 0) { entry_ms = ms; best_ms = ms; best_skin = skin0; }
+        else if (ms < best_ms) { best_ms = ms; best_skin = skin; }
+    }
+    int rc = nbx_batch_set_skin(b, best_skin);
+    if (rc) return rc;
+    b->accel_valid = false;   // new lists: the first run rebuilds them (a is still a(x))
+    (void)accel0;
+    if (out) { out[0] = best_skin; out[1] = best_ms; out[2] = entry_ms; }
     return NBX_OK;
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "nbx_batch_last_error": (C.c_char_p, [_P]),
     "nbx_batch_set_thermostat": (C.c_int, [_P, _I32, _I32, _D, _D]),
     "nbx_batch_set_skin": (C.c_int, [_P, _D]),
+    "nbx_batch_autotune": (C.c_int, [_P, _D, _I64, _DP]),
     "nbx_batch_set_state": (C.c_int, [_P, _DP, _DP]),
     "nbx_batch_get_state": (C.c_int, [_P, _DP, _DP]),
     "nbx_batch_run": (C.c_int, [_P, _D, _I64, _U64, _I64]),
@@ -413,6 +414,13 @@ class Batch:
 
     def set_skin(self, skin):
         self._check(self._lib.nbx_batch_set_skin(self._h, float(skin)))
+
+    def autotune(self, dt, trial_steps=200) -> dict:
+        """Pick the Verlet skin by timing (nbx_batch_autotune); the state is unchanged."""
+        out = np.zeros(3)
+        self._check(self._lib.nbx_batch_autotune(self._h, float(dt), int(trial_steps),
+                                                 dptr(out)))
+        return dict(skin=out[0], ms_per_step=out[1], entry_ms_per_step=out[2])
 
     def set_state(self, u, v=None):
         u = as_c3n(u, self.ntot)

@@ -126,3 +126,30 @@ def test_batch_argument_errors(nbx):
         b.set_thermostat(0, _lib.THERM_BERENDSEN, 1.0, 1.0)
     with pytest.raises(ValueError):
         b.run(0.005, 1)   # no state
+
+
+def test_batch_autotune_keeps_state_and_is_reproducible(nbx):
+    """The skin autotuner restores the state; a run after it equals a run of a fresh batch
+    given the chosen skin, bit for bit."""
+    print("[case] batch autotune", flush=True)
+    b, systems = make_batch([3, 5, 6])
+    for r in range(3):
+        b.set_thermostat(r, _lib.THERM_LANGEVIN, 1.0, 1.0)
+    U = np.concatenate([s.u for s in systems])
+    V = np.concatenate([s.v for s in systems])
+    b.set_state(U, V)
+    tuned = b.autotune(0.005, 50)
+    u0, v0 = b.get_state()
+    assert np.array_equal(u0, U) and np.array_equal(v0, V)
+    assert b.stats()["steps_taken"] == 0
+    assert tuned["skin"] in (0.2, 0.3, 0.4, 0.5, 0.6, 0.8) and tuned["ms_per_step"] > 0
+    b.run(0.005, 300, seed=4)
+    c, _ = make_batch([3, 5, 6])
+    for r in range(3):
+        c.set_thermostat(r, _lib.THERM_LANGEVIN, 1.0, 1.0)
+    c.set_skin(tuned["skin"])
+    c.set_state(U, V)
+    c.run(0.005, 300, seed=4)
+    ub, vb = b.get_state()
+    uc, vc = c.get_state()
+    assert np.array_equal(ub, uc) and np.array_equal(vb, vc)
