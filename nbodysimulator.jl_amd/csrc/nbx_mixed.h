@@ -7,9 +7,10 @@
 //   step(k)   : a_k = f(x_k, vh_k); v_k = vh_k + dt a_k / 2 (+ Langevin B half step,
 //               Andersen collisions); then the start of step k+1: Langevin A half step
 //               (registers only), x_{k+1} = x_k + dt v + dt^2 a_k / 2 into X[(k+1) & 1],
-//               vh_{k+1} = v + dt a_k / 2, wrapped copy into P[(k+1) & 1], binning into
-//               B[(k+1) & 1], skin test against the last build -> req[(k+1) & 1], and the
-//               Berendsen temperature of vh_{k+1} (last-block reduction).
+//               vh_{k+1} = v + dt a_k / 2, wrapped copy into P[(k+1) & 1], skin test
+//               against the last build -> req[(k+1) & 1], and the Berendsen temperature
+//               of vh_{k+1} (last-block reduction)
+//   bin(k+1)  : if req[(k+1) & 1]: P[(k+1) & 1] binned into B[(k+1) & 1]
 #pragma once
 #include "nbx_internal.h"
 
@@ -70,10 +71,7 @@ struct mx_io {
     double half_skin2;
     int* req_out;         // rebuild request for step k+1
     int* flags;           // [1] overflow
-    int* bin_count;       // bins B[(k+1) & 1]
-    int* bin_atoms;
-    int ccap, nc;
-    double inv_cs, L;
+    double L;             // cubic box side (wrapped copy)
     // Berendsen temperature of vh_{k+1}
     int need_T;
     double* red_part;     // one partial per block

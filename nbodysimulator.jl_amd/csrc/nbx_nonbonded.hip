@@ -11,10 +11,11 @@
 //   nbr            : Verlet list, interleaved for G lanes per atom: slot k of atom i is
 //                    at (k / G) * (n * G) + i * G + (k % G), so lane l of atom i reads
 //                    consecutive words with the other lanes of the wavefront
-//   B[2]           : fixed-capacity cell bins (count + ccap slots per cell), filled by
-//                    the drift of every step; a list is rebuilt from them when an atom
-//                    moved more than skin/2 since the last build (request flag set by
-//                    the drift, read by the build kernel, which exits at once otherwise)
+//   B[2]           : fixed-capacity cell bins (count + ccap slots per cell); a list is
+//                    rebuilt when an atom moved more than skin/2 since the last build:
+//                    the drift sets a request flag, and the binning launch after it and
+//                    the build launch of the next step read the flag and exit at once
+//                    when it is clear
 //
 // Pair law (per ordered pair, FP32): r2 = |r|^2; LJ (2 s^12 - s^6)/r2 * r with
 // s^2 = sigma^2/r2; Coulomb q_j / r^3 * r; RF q_j (1/r^3 - 2 k_rf) * r.  Per-atom
