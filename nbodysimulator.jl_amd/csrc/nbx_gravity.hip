@@ -83,6 +83,8 @@ k_grav_finish(const double* __restrict__ part, int chunks, int n, double G, mx_k
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double tval = 0.0;
     if (i < n) {
+        mx_atom A;
+        mx_load_atom(i, n, io, A);
         double s[3] = {0.0, 0.0, 0.0};
         for (int c = 0; c < chunks; ++c) {
             const double* pc = part + (size_t)c * 3 * n;
@@ -93,17 +95,17 @@ k_grav_finish(const double* __restrict__ part, int chunks, int n, double G, mx_k
         double ax = G * s[0], ay = G * s[1], az = G * s[2];
         if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
-            ax += cb * io.vh[i]; ay += cb * io.vh[n + i]; az += cb * io.vh[2 * n + i];
+            ax += cb * A.vh[0]; ay += cb * A.vh[1]; az += cb * A.vh[2];
         }
         io.a[i] = ax; io.a[n + i] = ay; io.a[2 * n + i] = az;
         if (kp.do_kick) {
             double v3[3];
-            mx_kick_atom(i, n, ax, ay, az, io.vh, io.ms, kp, v3);
+            mx_kick_atom(i, ax, ay, az, A, kp, v3);
             if (kp.therm == NBX_THERM_ANDERSEN &&
                 nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
                 mx_andersen_atom(i, kp.m0, kp, v3);
             io.v[i] = v3[0]; io.v[n + i] = v3[1]; io.v[2 * n + i] = v3[2];
-            if (io.do_drift) tval = mx_drift_atom(i, n, v3, ax, ay, az, kp, io);
+            if (io.do_drift) tval = mx_drift_atom(i, n, v3, ax, ay, az, A, kp, io);
         }
     }
     mx_epilogue(tval, io, n);

@@ -87,23 +87,48 @@ __device__ __forceinline__ void mx_resolve_step(mx_kick_params& kp) {
     if (kp.step_base != nullptr) kp.step = *kp.step_base + kp.step_off;
 }
 
+// Per-atom inputs of the end of a step, loaded together (mx_load_atom) so that a kernel
+// can issue them before its force loop: after the loop they would come in several
+// dependent rounds, since the step's own stores may alias them.
+struct mx_atom {
+    double m, q;          // mass, charge (0 without charges)
+    double vh[3];         // vh_k
+    double x[3];          // x_k (when drifting)
+    double xr[3];         // positions at the last list build (list mode, when drifting)
+};
+
+__device__ __forceinline__ void mx_load_atom(int64_t i, int64_t n, const mx_io& io,
+                                             mx_atom& A) {
+    A.m = io.ms[i];
+    A.q = io.qs != nullptr ? io.qs[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) A.vh[k] = io.vh[k * n + i];
+    if (io.do_drift) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A.x[k] = io.x_in[k * n + i];
+        if (io.xref != nullptr) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) A.xr[k] = io.xref[k * n + i];
+        }
+    }
+}
+
 // v = vh + dt (a/2), then the Langevin B half-step of step kp.step (Euler-Maruyama:
 // v = vh, the velocity the previous drift computed)
-__device__ __forceinline__ void mx_kick_atom(int64_t i, int64_t n, double ax, double ay,
-                                             double az, const double* __restrict__ vh,
-                                             const double* __restrict__ ms,
-                                             const mx_kick_params& kp, double* v3) {
+__device__ __forceinline__ void mx_kick_atom(int64_t i, double ax, double ay, double az,
+                                             const mx_atom& A, const mx_kick_params& kp,
+                                             double* v3) {
     if (kp.em) {  // Euler-Maruyama: vh already holds v of the step being finished
-        v3[0] = vh[i];
-        v3[1] = vh[n + i];
-        v3[2] = vh[2 * n + i];
+        v3[0] = A.vh[0];
+        v3[1] = A.vh[1];
+        v3[2] = A.vh[2];
         return;
     }
-    v3[0] = vh[i] + kp.dt * (0.5 * ax);
-    v3[1] = vh[n + i] + kp.dt * (0.5 * ay);
-    v3[2] = vh[2 * n + i] + kp.dt * (0.5 * az);
+    v3[0] = A.vh[0] + kp.dt * (0.5 * ax);
+    v3[1] = A.vh[1] + kp.dt * (0.5 * ay);
+    v3[2] = A.vh[2] + kp.dt * (0.5 * az);
     if (kp.therm == NBX_THERM_LANGEVIN) {
-        const double sig = kp.c2 * sqrt(kp.kb * kp.T / ms[i]);
+        const double sig = kp.c2 * sqrt(kp.kb * kp.T / A.m);
         double g[3];
         nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_LANGEVIN_B, g);
         for (int k = 0; k < 3; ++k) v3[k] = kp.c1 * v3[k] + sig * g[k];
@@ -130,16 +155,16 @@ __device__ __forceinline__ double mx_wrap(double p, double L) {
 }
 
 // wrapped copy of x_{k+1} and the skin test against the positions of the last build
-__device__ __forceinline__ void mx_store_position(int64_t i, int64_t n, double px, double py,
-                                                  double pz, double mi, const mx_io& io) {
+__device__ __forceinline__ void mx_store_position(int64_t i, double px, double py, double pz,
+                                                  const mx_atom& A, const mx_io& io) {
     if (io.pos_kind == 1) {
         const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
-        ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, io.qs != nullptr ? io.qs[i] : 0.0);
+        ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, A.q);
     } else if (io.pos_kind == 2) {
-        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)mi);
+        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)A.m);
     }
     if (io.xref != nullptr) {
-        const double ex = px - io.xref[i], ey = py - io.xref[n + i], ez = pz - io.xref[2 * n + i];
+        const double ex = px - A.xr[0], ey = py - A.xr[1], ez = pz - A.xr[2];
         if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
     }
 }
@@ -150,11 +175,12 @@ __device__ __forceinline__ void mx_store_position(int64_t i, int64_t n, double p
 // copies it to v).  Returns 0 (no thermostat temperature is needed).
 __device__ __forceinline__ double mx_em_atom(int64_t i, int64_t n, double vx, double vy,
                                              double vz, double ax, double ay, double az,
-                                             const mx_kick_params& kp, const mx_io& io) {
+                                             const mx_atom& A, const mx_kick_params& kp,
+                                             const mx_io& io) {
     const double dt = kp.dt;
-    const double px = io.x_in[i] + dt * vx;
-    const double py = io.x_in[n + i] + dt * vy;
-    const double pz = io.x_in[2 * n + i] + dt * vz;
+    const double px = A.x[0] + dt * vx;
+    const double py = A.x[1] + dt * vy;
+    const double pz = A.x[2] + dt * vz;
     io.x_out[i] = px;
     io.x_out[n + i] = py;
     io.x_out[2 * n + i] = pz;
@@ -163,19 +189,20 @@ __device__ __forceinline__ double mx_em_atom(int64_t i, int64_t n, double vx, do
     io.vh[i] = vx + dt * (ax - kp.em_gamma * vx) + kp.em_sigma * (kp.em_sqrt_dt * g[0]);
     io.vh[n + i] = vy + dt * (ay - kp.em_gamma * vy) + kp.em_sigma * (kp.em_sqrt_dt * g[1]);
     io.vh[2 * n + i] = vz + dt * (az - kp.em_gamma * vz) + kp.em_sigma * (kp.em_sqrt_dt * g[2]);
-    mx_store_position(i, n, px, py, pz, io.ms[i], io);
+    mx_store_position(i, px, py, pz, A, io);
     return 0.0;
 }
 
 // Start of step kp.step + 1 for atom i from v (after the previous step's end) and a:
-// Langevin A half-step in registers, drift, half kick, wrapped copy, bins, skin test.
+// Langevin A half-step in registers, drift, half kick, wrapped copy, skin test.
 // Returns m vh^2 (the Berendsen temperature input).
 __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const double* v3,
                                                 double ax, double ay, double az,
-                                                const mx_kick_params& kp, const mx_io& io) {
+                                                const mx_atom& A, const mx_kick_params& kp,
+                                                const mx_io& io) {
     double vx = v3[0], vy = v3[1], vz = v3[2];
-    const double mi = io.ms[i];
-    if (kp.em) return mx_em_atom(i, n, vx, vy, vz, ax, ay, az, kp, io);
+    const double mi = A.m;
+    if (kp.em) return mx_em_atom(i, n, vx, vy, vz, ax, ay, az, A, kp, io);
     if (kp.therm == NBX_THERM_LANGEVIN) {
         const double sig = kp.c2 * sqrt(kp.kb * kp.T / mi);
         double g[3];
@@ -185,9 +212,9 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
         vz = kp.c1 * vz + sig * g[2];
     }
     const double dt = kp.dt, dtsq = dt * dt;
-    const double px = io.x_in[i] + dt * vx + dtsq * (0.5 * ax);
-    const double py = io.x_in[n + i] + dt * vy + dtsq * (0.5 * ay);
-    const double pz = io.x_in[2 * n + i] + dt * vz + dtsq * (0.5 * az);
+    const double px = A.x[0] + dt * vx + dtsq * (0.5 * ax);
+    const double py = A.x[1] + dt * vy + dtsq * (0.5 * ay);
+    const double pz = A.x[2] + dt * vz + dtsq * (0.5 * az);
     io.x_out[i] = px;
     io.x_out[n + i] = py;
     io.x_out[2 * n + i] = pz;
@@ -195,7 +222,7 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
     io.vh[i] = hx;
     io.vh[n + i] = hy;
     io.vh[2 * n + i] = hz;
-    mx_store_position(i, n, px, py, pz, mi, io);
+    mx_store_position(i, px, py, pz, A, io);
     return mi * (hx * hx + hy * hy + hz * hz);
 }
 

@@ -68,16 +68,16 @@ __device__ __forceinline__ double lane_sum(double x) {
 // drift.  Returns m vh^2 of the next step (0 when not drifting).
 template <bool LJ, int EL, bool WATER>
 __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* F,
-                                            const mx_pair_params& pp,
+                                            const mx_atom& A, const mx_pair_params& pp,
                                             const mx_kick_params& kp, const mx_io& io) {
-    const double mi = io.ms[i];
+    const double mi = A.m;
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (LJ) {
         const double c = pp.lj_c / mi;
         ax += c * F[0]; ay += c * F[1]; az += c * F[2];
     }
     if (EL != 0) {
-        const double c = pp.el_k * io.qs[i] / mi;
+        const double c = pp.el_k * A.q / mi;
         ax += c * F[3]; ay += c * F[4]; az += c * F[5];
     }
     if (WATER) {  // bonded terms and the end of the step are done per molecule
@@ -86,18 +86,18 @@ __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* 
     }
     if (mx_therm_force(kp)) {
         const double cb = mx_berendsen_coeff(kp);
-        ax += cb * io.vh[i]; ay += cb * io.vh[n + i]; az += cb * io.vh[2 * n + i];
+        ax += cb * A.vh[0]; ay += cb * A.vh[1]; az += cb * A.vh[2];
     }
     io.a[i] = ax; io.a[n + i] = ay; io.a[2 * n + i] = az;
     if (!kp.do_kick) return 0.0;
     double v3[3];
-    mx_kick_atom(i, n, ax, ay, az, io.vh, io.ms, kp, v3);
+    mx_kick_atom(i, ax, ay, az, A, kp, v3);
     if (kp.therm == NBX_THERM_ANDERSEN &&
         nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
         mx_andersen_atom(i, kp.m0, kp, v3);
     io.v[i] = v3[0]; io.v[n + i] = v3[1]; io.v[2 * n + i] = v3[2];
     if (!io.do_drift) return 0.0;
-    return mx_drift_atom(i, n, v3, ax, ay, az, kp, io);
+    return mx_drift_atom(i, n, v3, ax, ay, az, A, kp, io);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,6 +119,16 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
         const double4 pi = pos[i];
         const int cnt = nnb[i];
         const bool io_ = !WATER || (i % 3) == 0;
+        // the atom's inputs to the end of the step, loaded while the pairs are summed
+        mx_atom A;
+        if (lane == 0) {
+            if (WATER) {
+                A.m = io.ms[i];
+                A.q = EL != 0 ? io.qs[i] : 0.0;
+            } else {
+                mx_load_atom(i, n, io, A);
+            }
+        }
         nbx_kahan acc[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[q].init();
@@ -157,7 +167,7 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
         double F[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) F[q] = lane_sum<G>(acc[q].value());
-        if (lane == 0) tval = nb_finish<LJ, EL, WATER>(i, n, F, pp, kp, io);
+        if (lane == 0) tval = nb_finish<LJ, EL, WATER>(i, n, F, A, pp, kp, io);
     }
     if (!WATER) mx_epilogue(tval, io, n);
 }
@@ -209,7 +219,14 @@ k_nb_allpairs(int n, mx_pair_params pp, nbx_bcv bc, mx_kick_params kp, mx_io io)
         double F[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) F[q] = acc[q].value();
-        tval = nb_finish<LJ, EL, WATER>(i, n, F, pp, kp, io);
+        mx_atom A;
+        if (WATER) {
+            A.m = io.ms[i];
+            A.q = EL != 0 ? io.qs[i] : 0.0;
+        } else {
+            mx_load_atom(i, n, io, A);
+        }
+        tval = nb_finish<LJ, EL, WATER>(i, n, F, A, pp, kp, io);
     }
     if (!WATER) mx_epilogue(tval, io, n);
 }
@@ -275,8 +292,10 @@ __global__ void __launch_bounds__(256) k_mx_drift(int n, mx_kick_params kp, mx_i
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double tval = 0.0;
     if (i < n) {
+        mx_atom A;
+        mx_load_atom(i, n, io, A);
         const double v3[3] = {io.v[i], io.v[n + i], io.v[2 * n + i]};
-        tval = mx_drift_atom(i, n, v3, io.a[i], io.a[n + i], io.a[2 * n + i], kp, io);
+        tval = mx_drift_atom(i, n, v3, io.a[i], io.a[n + i], io.a[2 * n + i], A, kp, io);
     }
     mx_epilogue(tval, io, n);
 }
@@ -635,6 +654,8 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
     __syncthreads();
     double tval = 0.0;
     if (act) {
+        mx_atom A;
+        mx_load_atom(i, n, io, A);
         const int w = t % 3;               // 0: O, 1: H1, 2: H2
         const int b = t - w;               // the molecule's O in this block
         const int o = i - w;
@@ -650,7 +671,7 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
         } else {
             mol_bond(r[w], r[0], rOH, kbond, f);
         }
-        const double mi = io.ms[i];
+        const double mi = A.m;
         const double im = 1.0 / mi;
         double acc[3];
 #pragma unroll
@@ -684,19 +705,19 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
         }
         if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
-            for (int k = 0; k < 3; ++k) acc[k] += cb * io.vh[(size_t)k * n + i];
+            for (int k = 0; k < 3; ++k) acc[k] += cb * A.vh[k];
         }
         for (int k = 0; k < 3; ++k) io.a[(size_t)k * n + i] = acc[k];
         if (kp.do_kick) {
             double v3[3];
-            mx_kick_atom(i, n, acc[0], acc[1], acc[2], io.vh, io.ms, kp, v3);
+            mx_kick_atom(i, acc[0], acc[1], acc[2], A, kp, v3);
             if (kp.therm == NBX_THERM_ANDERSEN &&
                 nbx_rng_uniform(kp.seed, o, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
                 mx_andersen_atom(i, mi, kp, v3);
             io.v[i] = v3[0];
             io.v[n + i] = v3[1];
             io.v[2 * n + i] = v3[2];
-            if (io.do_drift) tval = mx_drift_atom(i, n, v3, acc[0], acc[1], acc[2], kp, io);
+            if (io.do_drift) tval = mx_drift_atom(i, n, v3, acc[0], acc[1], acc[2], A, kp, io);
         }
     }
     mx_epilogue(tval, io, n);
