@@ -10,7 +10,8 @@ with the state resident in HBM before the timed region starts.
 
 The line also carries:
   roofline     -- the dominant kernel (k_grav_tiled): 20 flop per interaction divided by
-                  its average duration, timed with HIP events on the library's stream
+                  its average duration over the timed steps (a HIP event pair around each
+                  of its launches, on the library's stream)
   cpu_baseline -- the FP64 oracle (a restatement of the reference's per-atom loop) on
                   one host core, timed on a bounded sample of targets
   secondary    -- Lennard-Jones N = 32,000 (FCC, rho* = 0.8442, T* = 0.9, r_c = 2.5) and
@@ -99,16 +100,19 @@ def mixed_accel(ctx, u):
     return (vf[1] - vf[0]) / dt
 
 
-def gravity_issue_model(ctx, pairs, kernel_ms):
+def gravity_issue_model(ctx, pairs, kernel_ms, equal_masses):
     """Issue-limited time of k_grav_tiled from the measured issue rates of its
     instruction kinds (nbx_valu_probe, wave64 instructions per SIMD per ns): per pair
-    3 v_sub + 3 v_mul at the v_mul rate, 6 FMA at the v_fmac rate, one v_rsq_f32."""
+    3 v_sub + 3 v_mul (2 when every body has the same mass: the kernel then leaves the
+    mass out of the pair term) at the v_mul rate, 6 FMA at the v_fmac rate, one
+    v_rsq_f32."""
     r_fma, r_mul, r_rsq = (ctx.valu_probe(k, 5) for k in (0, 1, 2))
     simds = 4 * 256
-    ns_per_wave_pair = 6.0 / r_mul + 6.0 / r_fma + 1.0 / r_rsq     # per 64 pairs, per SIMD
+    muls = 2 if equal_masses else 3
+    ns_per_wave_pair = (3.0 + muls) / r_mul + 6.0 / r_fma + 1.0 / r_rsq  # per 64 pairs, per SIMD
     bound_ms = pairs / 64.0 * ns_per_wave_pair / simds * 1e-6
     return {"fmac_per_simd_ns": r_fma, "mul_per_simd_ns": r_mul, "rsq_per_simd_ns": r_rsq,
-            "instructions_per_pair": "3 sub, 3 mul, 6 fma, 1 v_rsq_f32",
+            "instructions_per_pair": f"3 sub, {muls} mul, 6 fma, 1 v_rsq_f32",
             "issue_bound_ms": bound_ms, "frac_of_issue_bound": bound_ms / kernel_ms}
 
 
@@ -129,14 +133,20 @@ def bench_gravity(args, lib, gen):
     tuned = ctx.autotune(dt, 20)
     ke0, _, pg0 = ctx.energies()
     ctx.vv_run(dt, args.warmup, seed=1)
+    # HIP event pairs around every k_grav_tiled launch of the timed steps (on the
+    # library's stream; the step graphs are off while they record: at 1 ms per kernel the
+    # host stays ahead of the GPU either way)
+    ctx.set_kernel_timing(True)
     t0 = time.perf_counter()
     ctx.vv_run(dt, args.steps, seed=1)          # synchronous: returns after the stream drains
     elapsed = time.perf_counter() - t0
+    kms, klaunches, kname = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
     ke1, _, pg1 = ctx.energies()
     e0, e1 = ke0 + pg0, ke1 + pg1
     pairs = n * (n - 1)
     value = pairs * args.steps / elapsed
-    kms, kname = ctx.time_force_kernel(20)
+    kms_b2b, _ = ctx.time_force_kernel(20)
     # the same run continued to 10k steps for the long-run energy drift
     drift10k = None
     if not args.no_secondary and not args.no_long_drift:
@@ -144,7 +154,7 @@ def bench_gravity(args, lib, gen):
         ke2, _, pg2 = ctx.energies()
         drift10k = abs((ke2 + pg2) - e0) / abs(e0)
     achieved = FLOP_GRAV * pairs / (kms * 1e-3) / 1e12
-    issue = gravity_issue_model(ctx, pairs, kms)
+    issue = gravity_issue_model(ctx, pairs, kms, bool(np.all(m == m[0])))
     sample_pairs = len(targets) * (n - 1)
     cpu_rate = sample_pairs / cpu_s
     return {
@@ -165,6 +175,8 @@ def bench_gravity(args, lib, gen):
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic_from_profiles(kname),
                      "kernel_avg_ms": kms,
+                     "kernel_launches_timed": klaunches,
+                     "kernel_avg_ms_back_to_back": kms_b2b,
                      "flop_per_launch": FLOP_GRAV * pairs,
                      "issue_model": issue},
         "cpu_baseline": {"value": cpu_rate, "unit": "pair interactions/s", "cores": 1,

@@ -180,6 +180,15 @@ int nbx_rdf(nbx_ctx* ctx, const double* frames, int64_t nframes, int64_t maxbin,
  * receives a static string naming the kernel. */
 int nbx_time_force_kernel(nbx_ctx* ctx, int32_t iters, double* avg_ms,
                           const char** kernel_name);
+/* Measurement support (no reference counterpart): with enable != 0, every later
+ * mixed-precision nbx_vv_run records a pair of HIP events on the context's stream around
+ * each launch of its force kernel (k_grav_tiled; k_nb_list / k_nb_allpairs, whose launch
+ * also ends the step and drifts) and runs without captured step graphs; the counters are
+ * reset by every call.  nbx_get_kernel_timing returns the average duration (ms) and the
+ * number of launches timed since, and the kernel's name (static string). */
+int nbx_set_kernel_timing(nbx_ctx* ctx, int32_t enable);
+int nbx_get_kernel_timing(nbx_ctx* ctx, double* avg_ms, int64_t* launches,
+                          const char** kernel_name);
 /* Ordered in-cutoff pairs of the current state (pair-interaction count of one force
  * evaluation; n(n-1) for gravity). */
 int nbx_count_pairs(nbx_ctx* ctx, int64_t* pairs);

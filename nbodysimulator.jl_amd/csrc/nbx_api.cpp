@@ -77,6 +77,8 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     c->n = n;
     c->sys = system_kind;
     c->h_ms.assign(masses, masses + n);
+    c->equal_masses = n > 0 && std::all_of(masses, masses + n,
+                                           [&](double m) { return m == masses[0]; });
     if (charges) {
         c->h_qs.assign(charges, charges + n);
         c->has_q = true;
@@ -789,6 +791,24 @@ extern "C" int nbx_time_force_kernel(nbx_ctx* c, int32_t iters, double* avg_ms,
     if (c->prec != NBX_PREC_MIXED)
         return nbx_fail(c, NBX_ERR_UNSUPPORTED, "kernel timing is provided for the mixed path");
     return mx_time_force(c, iters, avg_ms, name);
+}
+
+extern "C" int nbx_set_kernel_timing(nbx_ctx* c, int32_t enable) {
+    if (!c) return NBX_ERR_INVALID;
+    c->kt_on = enable != 0;
+    c->kt_sum_ms = 0.0;
+    c->kt_launches = 0;
+    c->kt_used = 0;
+    return NBX_OK;
+}
+
+extern "C" int nbx_get_kernel_timing(nbx_ctx* c, double* avg_ms, int64_t* launches,
+                                     const char** name) {
+    if (!c || !avg_ms || !launches) return NBX_ERR_INVALID;
+    *launches = c->kt_launches;
+    *avg_ms = c->kt_launches > 0 ? c->kt_sum_ms / (double)c->kt_launches : 0.0;
+    if (name) *name = mx_force_kernel_name(c);
+    return NBX_OK;
 }
 
 extern "C" int nbx_copy_probe(nbx_ctx* c, int64_t bytes, int32_t iters, double* avg_ms) {

@@ -111,12 +111,20 @@ struct nbx_ctx {
     int grav_chunks = 1;
     int grav_tpt_set = 0;     // targets per thread of the gravity kernel (0: by size)
     int grav_chunks_set = 0;  // source chunks of the gravity kernel (0: by size)
+    bool equal_masses = false;  // every body has mass h_ms[0] (gravity takes m out of the sum)
     nbx_nlist nl;
     int64_t* d_step = nullptr;       // absolute step base read by replayed graphs
     hipGraphExec_t gexec = nullptr;  // captured block of mixed-path steps
     nbx_graph_key gkey;
     int64_t version = 0;             // bumped by every configuration change
     int graph_steps = 50;            // steps per captured graph (0: no graphs)
+    // force-kernel timing over the steps of a run (measurement): a pair of HIP events
+    // around every force-kernel launch of the mixed path, summed after each chunk
+    bool kt_on = false;
+    std::vector<hipEvent_t> kt_ev;
+    size_t kt_used = 0;
+    double kt_sum_ms = 0.0;
+    int64_t kt_launches = 0;
     bool state_set = false;
     bool accel_valid = false; // d_a holds f(d_x, d_v) for the current state
     int64_t steps_taken = 0;
@@ -202,6 +210,7 @@ int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d
 
 // ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name);
+const char* mx_force_kernel_name(nbx_ctx* c);
 int mx_valu_probe(nbx_ctx* c, int kind, int reps, double* rate);
 int mx_copy_probe(nbx_ctx* c, int64_t bytes, int iters, double* avg_ms);
 int mx_autotune(nbx_ctx* c, double dt, int trial_steps, double* out, int nout);

@@ -271,20 +271,34 @@ void after_drift(nbx_ctx* c, const mx_plan& p, int par_out) {
                       bin_atoms(c, par_out));
 }
 
+// the next pair of force-kernel timing events, when timing is on and the pool has room
+// (the pool holds two per step of the longest chunk)
+void timing_events(nbx_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+    *e0 = *e1 = nullptr;
+    if (!c->kt_on || c->kt_used + 2 > c->kt_ev.size()) return;
+    *e0 = c->kt_ev[c->kt_used++];
+    *e1 = c->kt_ev[c->kt_used++];
+}
+
 // force evaluation at x in X[par] (list: P[par]); with kp.do_kick the step's end, with
 // do_drift the start of the next step
-void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par, int do_drift) {
+void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par, int do_drift,
+                 bool timed = false) {
     hipStream_t s = c->stream;
     const mx_io io = make_io(c, p, par, do_drift, kp.dt);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) timing_events(c, &e0, &e1);
     if (p.kind == PLAN_GRAVITY) {
-        mx_launch_gravity(s, c, pbuf(c, par), p.G, kp, io, 0);
+        mx_launch_gravity(s, c, pbuf(c, par), p.G, kp, io, 0, e0, e1);
     } else if (p.kind == PLAN_MAGNETIC) {
         mx_launch_magnetic(s, c, pbuf(c, par), p.mu, kp, io, 0);
     } else {
         mx_nb_launch nb = p.nb;
         nb.lanes = c->nl.lanes;
         int* clear = p.nb.mode_list ? req_flag(c, par) : nullptr;
+        if (e0) (void)hipEventRecord(e0, s);
         mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io, clear);
+        if (e1) (void)hipEventRecord(e1, s);
         if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
     }
     if (do_drift) after_drift(c, p, par ^ 1);
@@ -415,6 +429,12 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
     backup4 bk;
     NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * (4 * n3 + 16)));
     const int64_t chunk_max = 500;
+    while (c->kt_on && c->kt_ev.size() < 2 * (size_t)chunk_max) {
+        hipEvent_t e = nullptr;
+        NBX_HIP(c, hipEventCreate(&e));
+        c->kt_ev.push_back(e);
+    }
+    c->kt_used = 0;
     // steps k = 1..count of the chunk, absolute step *d_step + k; then *d_step += count
     auto enqueue_steps = [&](int64_t k0, int64_t count) {
         for (int64_t k = k0 + 1; k <= k0 + count; ++k) {
@@ -423,7 +443,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
             kp.step_base = c->d_step;
             kp.step_off = k - k0;
             launch_build(c, p, par);
-            launch_step(c, p, kp, par, 1);
+            launch_step(c, p, kp, par, 1, true);
         }
         mx_launch_step_advance(s, c->d_step, count, 0);
     };
@@ -496,7 +516,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
             }
             const int gs = c->graph_steps;
             int64_t k0 = 0;
-            if (gs > 0 && (gs % 2) == 0 && chunk >= gs) {
+            if (gs > 0 && (gs % 2) == 0 && chunk >= gs && !c->kt_on) {
                 hipGraphExec_t ge = nullptr;
                 if ((rc = graph_for(gs, &ge))) return rc;
                 for (; k0 + gs <= chunk; k0 += gs) NBX_HIP(c, hipGraphLaunch(ge, s));
@@ -508,7 +528,18 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                                           hipMemcpyDeviceToDevice, s));
             int h[8];
             if ((rc = read_flags(c, h))) return rc;
-            if (!h[1]) break;
+            if (!h[1]) {
+                // the chunk stands: add its force-kernel times (the stream has drained)
+                for (size_t k = 0; k + 1 < c->kt_used; k += 2) {
+                    float ms = 0.f;
+                    NBX_HIP(c, hipEventElapsedTime(&ms, c->kt_ev[k], c->kt_ev[k + 1]));
+                    c->kt_sum_ms += ms;
+                    ++c->kt_launches;
+                }
+                c->kt_used = 0;
+                break;
+            }
+            c->kt_used = 0;   // replayed: its timings are dropped
             // overflow: restore and replay with larger lists
             double* dst[4] = {c->d_x, c->d_v, c->d_a, c->d_vh};
             for (int k = 0; k < 4; ++k)
@@ -536,6 +567,15 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         }
     }
     return NBX_OK;
+}
+
+// the step's dominant (force) kernel
+const char* mx_force_kernel_name(nbx_ctx* c) {
+    mx_plan p;
+    if (!make_plan(c, p, nullptr)) return "";
+    if (p.kind == PLAN_GRAVITY) return "k_grav_tiled";
+    if (p.kind == PLAN_MAGNETIC) return "k_mag_tiled";
+    return p.nb.mode_list ? "k_nb_list" : "k_nb_allpairs";
 }
 
 int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
@@ -566,12 +606,7 @@ int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *avg_ms = (double)ms / iters;
-    if (name) {
-        if (p.kind == PLAN_GRAVITY) *name = "k_grav_tiled";
-        else if (p.kind == PLAN_MAGNETIC) *name = "k_mag_tiled";
-        else if (p.nb.mode_list) *name = "k_nb_list";
-        else *name = "k_nb_allpairs";
-    }
+    if (name) *name = mx_force_kernel_name(c);
     // the timed launches overwrote a with the force of the current positions alone
     // (water: the non-bonded part; Berendsen: with the look-ahead velocities), so the
     // next run re-evaluates a; x and v are unchanged
@@ -612,6 +647,8 @@ int mx_count_pairs(nbx_ctx* c, int64_t* pairs) {
 }
 
 void mx_release(nbx_ctx* c) {
+    for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
+    c->kt_ev.clear();
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
     if (c->d_step) (void)hipFree(c->d_step);

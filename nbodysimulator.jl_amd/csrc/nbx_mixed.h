@@ -317,7 +317,8 @@ void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& 
                            const void* pos, const mx_pair_params& pp,
                            unsigned long long* d_count);
 void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double G,
-                       const mx_kick_params& kp, const mx_io& io, int force_only_kernel);
+                       const mx_kick_params& kp, const mx_io& io, int force_only_kernel,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // all-pairs magnetic dipoles: force into c->d_part (one chunk), then k_grav_finish
 void mx_launch_magnetic(hipStream_t s, const nbx_ctx* c, const void* pos, double mu_4pi,
                         const mx_kick_params& kp, const mx_io& io, int force_only_kernel);

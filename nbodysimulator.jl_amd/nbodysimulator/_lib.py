@@ -65,6 +65,8 @@ SIGNATURES = {
     "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
+    "nbx_set_kernel_timing": (C.c_int, [_P, _I32]),
+    "nbx_get_kernel_timing": (C.c_int, [_P, _DP, C.POINTER(_I64), C.POINTER(C.c_char_p)]),
     "nbx_copy_probe": (C.c_int, [_P, _I64, _I32, _DP]),
     "nbx_valu_probe": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
     "nbx_get_stats": (C.c_int, [_P, C.POINTER(_I64), _I32]),
@@ -307,6 +309,20 @@ class Context:
         gr = np.empty(maxbin)
         self._check(self._lib.nbx_rdf(self._h, dptr(f), f.shape[0], maxbin, dptr(rs), dptr(gr)))
         return rs, gr
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        """Time every force-kernel launch of later mixed-path runs (HIP event pairs on the
+        context's stream; no step graphs while on).  Resets the counters."""
+        self._check(self._lib.nbx_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def kernel_timing(self):
+        """(average ms, launches, kernel name) of the force launches timed so far."""
+        ms = C.c_double()
+        cnt = _I64()
+        name = C.c_char_p()
+        self._check(self._lib.nbx_get_kernel_timing(self._h, C.byref(ms), C.byref(cnt),
+                                                    C.byref(name)))
+        return ms.value, cnt.value, name.value.decode()
 
     def time_force_kernel(self, iters: int):
         ms = C.c_double()

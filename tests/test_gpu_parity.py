@@ -260,6 +260,22 @@ def test_mixed_force_error_lj_water_gravity(c1):
         assert relerr(aw, ow.rhs(w.u)) <= MIXED_TOL
 
 
+def test_mixed_gravity_equal_and_unequal_masses():
+    # with one mass for all bodies the tiles sum r_ij / r^3 and the finish applies G m in
+    # FP64; otherwise m_j enters every pair term.  Both against the FP64 oracle, with a
+    # common mass that is not a power of two and with masses spread over a factor 3.
+    u, _, m = gen.plummer(4096, seed=5)
+    rng = np.random.default_rng(11)
+    for masses, G in ((m, 1.0), (np.full_like(m, 0.3), 0.7),
+                      (m * rng.uniform(0.5, 1.5, size=m.shape), 1.0)):
+        cg = _lib.Context(4096, _lib.SYS_POTENTIAL, masses)
+        cg.add_gravitational(G)
+        cg.set_precision(_lib.PREC_MIXED)
+        ag = _mixed_accel(cg, u, np.zeros_like(u))
+        ref = O.System(masses).add(O.TERM_GRAV, G).rhs(u)
+        assert relerr(ag, ref) <= MIXED_TOL, (masses[:2], G)
+
+
 def test_mixed_all_pairs_mode_matches(c1):
     # periodic lo/hi wrapping is not minimum image: the mixed path uses all pairs
     ctx, orc = lj_pair(864, c1.L, prec=_lib.PREC_MIXED, bc_kind=_lib.BC_PERIODIC)
@@ -307,6 +323,32 @@ def test_mixed_count_pairs_and_kernel_timer(c1):
     assert ms > 0 and name == "k_nb_list"
     # issue-rate calibration: an independent v_rsq_f32 stream issues slower than v_fmac_f32
     assert ctx.valu_probe(0, 2) > ctx.valu_probe(2, 2) > 0
+
+
+def test_kernel_timing_over_a_run(c1):
+    # one timed launch per step (graphs off while timing), and the same trajectory bit for
+    # bit as the untimed (graph-replayed) run
+    u, v, m = gen.plummer(2048, seed=4)
+    cases = []
+    g = _lib.Context(2048, _lib.SYS_POTENTIAL, m)
+    g.add_gravitational(1.0)
+    cases.append((g, u, v, 120, "k_grav_tiled"))
+    lj, _ = lj_pair(864, c1.L, prec=_lib.PREC_MIXED)
+    cases.append((lj, c1.u, c1.v, 120, "k_nb_list"))
+    for ctx, u0, v0, steps, kname in cases:
+        ctx.set_precision(_lib.PREC_MIXED)
+        ctx.set_state(u0, v0)
+        ctx.vv_run(0.001, steps)
+        ref = ctx.get_state()
+        ctx.set_state(u0, v0)
+        ctx.set_kernel_timing(True)
+        ctx.vv_run(0.001, steps)
+        ms, launches, name = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        got = ctx.get_state()
+        assert name == kname and launches == steps and ms > 0, (name, launches, ms)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), kname
+        assert ctx.kernel_timing()[1] == 0
 
 
 # ---------------------------------------------------------------------------
