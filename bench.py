@@ -49,15 +49,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+def traffic_from_profiles(kernel: str, shape: str):
+    """HBM bytes per launch of `kernel` at tile shape `shape` ("<targets per thread>x
+    <source chunks>") from the committed rocprofv3 counter summary (None if absent)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+        return d.get("shapes", {}).get(shape, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -173,7 +174,8 @@ def bench_gravity(args, lib, gen):
                                  "ms_per_step_default": tuned["entry_ms_per_step"]}},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": traffic_from_profiles(kname),
+                     "traffic": traffic_from_profiles(
+                         kname, f"{int(tuned['shape0'])}x{int(tuned['shape1'])}"),
                      "kernel_avg_ms": kms,
                      "kernel_launches_timed": klaunches,
                      "kernel_avg_ms_back_to_back": kms_b2b,
