@@ -403,7 +403,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
         t0 = time.perf_counter()
         losses = model.train(iters, batch=8192, lr=1e-3, seed=7)
         t_train = time.perf_counter() - t0
-        # the plain integrator judges the strided step on trajectories never trained on
+        # the plain integrator scores the strided step on trajectories never trained on
         heldm.set(*model.get())
         mse, base = heldm.evaluate(0, heldm.num_samples)
         info = {"trajectories": ntr, "heldout_trajectories": nheld, "steps_each": traj_steps,
@@ -425,7 +425,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
                 "energy_drift_rel_10k": abs(rep["E"] - rep["E0"]) / abs(rep["E0"]),
                 "fraction_of_time_strided": rep["strided_steps"] * stride / nsteps_dt}, u
 
-    # plain velocity Verlet at dt: the reference point and the judge
+    # plain velocity Verlet at dt: the reference point and the yardstick
     plain, s = ctx_for(4242)
     plain.autotune(dt, 200)
     ke0, pe0 = plain.learn_energy()

@@ -201,7 +201,7 @@ def test_drift_monitor_falls_back_to_plain_vv(nbx):
 
 def test_trained_network_beats_the_uncorrected_stride(nbx):
     """The network's gain: on held-out trajectories its corrected 8-dt step is closer to
-    eight plain steps (the judge) than the uncorrected 8-dt step, along the force."""
+    eight plain steps (the yardstick) than the uncorrected 8-dt step, along the force."""
     print("[case] train on N=864 trajectories, evaluate held out", flush=True)
     m = _lib.Model(3)
     for seed in range(6):
