@@ -204,7 +204,7 @@ int nbx_copy_probe(nbx_ctx* ctx, int64_t bytes, int32_t iters, double* avg_ms);
 int nbx_valu_probe(nbx_ctx* ctx, int32_t kind, int32_t reps, double* rate);
 /* Autotuner (no reference counterpart): times `trial_steps` mixed-path steps of length dt
  * from the current state for every candidate launch shape and keeps the fastest -- list
- * plans: lanes per atom {2, 4, 8, 16} x Verlet skin {0.06 .. 0.40} x cutoff; gravity:
+ * plans: lanes per atom {2, 4, 8, 16, 32} x Verlet skin {0.06 .. 0.40} x cutoff; gravity:
  * targets per thread {1, 2, 4} x source chunks {1 .. 64}.  The state and step count are
  * restored.  out (nout <= 5): [0] lanes | targets per thread, [1] skin | chunks, [2] best
  * ms per step, [3] ms per step of the configuration on entry, [4] candidates timed. */

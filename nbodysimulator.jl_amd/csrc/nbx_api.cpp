@@ -88,7 +88,7 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     if (const char* g = std::getenv("NBX_GRAPH_STEPS")) c->graph_steps = std::max(0, std::atoi(g));
     if (const char* g = std::getenv("NBX_LANES")) {
         const int l = std::atoi(g);
-        if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16) c->nl.lanes = l;
+        if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32) c->nl.lanes = l;
     }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

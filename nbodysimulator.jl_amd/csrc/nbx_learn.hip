@@ -444,6 +444,7 @@ void lr_launch_features(hipStream_t s, int lanes, const lr_feat_params& fp) {
     case 2: hipLaunchKernelGGL(k_lr_features<2>, grid, dim3(256), 0, s, fp); break;
     case 4: hipLaunchKernelGGL(k_lr_features<4>, grid, dim3(256), 0, s, fp); break;
     case 16: hipLaunchKernelGGL(k_lr_features<16>, grid, dim3(256), 0, s, fp); break;
+    case 32: hipLaunchKernelGGL(k_lr_features<32>, grid, dim3(256), 0, s, fp); break;
     default: hipLaunchKernelGGL(k_lr_features<8>, grid, dim3(256), 0, s, fp); break;
     }
 }
@@ -720,6 +721,7 @@ void lr_launch_energy(hipStream_t s, int lanes, const lr_energy_params& ep, doub
     case 2: hipLaunchKernelGGL(k_lr_energy<2>, dim3(blocks), dim3(256), 0, s, ep); break;
     case 4: hipLaunchKernelGGL(k_lr_energy<4>, dim3(blocks), dim3(256), 0, s, ep); break;
     case 16: hipLaunchKernelGGL(k_lr_energy<16>, dim3(blocks), dim3(256), 0, s, ep); break;
+    case 32: hipLaunchKernelGGL(k_lr_energy<32>, dim3(blocks), dim3(256), 0, s, ep); break;
     default: hipLaunchKernelGGL(k_lr_energy<8>, dim3(blocks), dim3(256), 0, s, ep); break;
     }
     hipLaunchKernelGGL(k_lr_energy_final, dim3(1), dim3(256), 0, s, ep.part, blocks, out);

@@ -1010,7 +1010,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
 // timing.  Each candidate runs `trial` velocity-Verlet steps from the current state after
 // a warm-up run long enough to capture its step graph; the state, the step count and the
 // integrator's inputs are restored after every trial, and the fastest candidate is kept.
-//   list plans   : lanes per atom {2, 4, 8, 16} x skin {0.06 .. 0.40} x cutoff
+//   list plans   : lanes per atom {2, 4, 8, 16, 32} x skin {0.06 .. 0.40} x cutoff
 //   gravity      : targets per thread {1, 2, 4} x source chunks {1, 2, 4, ..., 64}
 // out: [0] lanes | targets per thread, [1] skin | chunks, [2] best ms per step,
 //      [3] ms per step of the configuration found on entry, [4] candidates timed.
@@ -1098,10 +1098,10 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         best_ms = base_ms;
         best0 = lanes0;
         best1 = c->nl.skin;
-        const int lanes_c[4] = {2, 4, 8, 16};
+        const int lanes_c[5] = {2, 4, 8, 16, 32};
         const double skin_c[7] = {0.06, 0.09, 0.12, 0.18, 0.24, 0.32, 0.40};
         bool set_best = false;
-        for (int li = 0; li < 4; ++li)
+        for (int li = 0; li < 5; ++li)
             for (int si = 0; si < 7; ++si) {
                 c->nl.lanes = lanes_c[li];
                 c->skin = skin_c[si] * p.cutoff;

@@ -255,6 +255,7 @@ static void launch_nb(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c, co
         case 2: launch_list<2, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
         case 4: launch_list<4, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
         case 16: launch_list<16, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
+        case 32: launch_list<32, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
         default: launch_list<8, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
         }
     } else {

@@ -325,6 +325,35 @@ def test_mixed_count_pairs_and_kernel_timer(c1):
     assert ctx.valu_probe(0, 2) > ctx.valu_probe(2, 2) > 0
 
 
+def test_every_lane_count_gives_the_same_physics(c1, monkeypatch):
+    # the Verlet list is interleaved for G lanes per atom; every G the autotuner can pick
+    # (and the learned stride's descriptor and energy kernels over the same list) must
+    # read it correctly: forces against the FP64 oracle, the same pair count, and the
+    # descriptor of the learned stride against its oracle
+    s = gen.lj_liquid(5, rho=0.8442, T=0.9, seed=3, jitter=0.03)
+    orc = O.System(np.ones(500), bc=O.make_bc(O.BC_CUBIC, s.L)).add(O.TERM_LJ, 1.0, 1.0, 2.5)
+    a_ref = orc.rhs(s.u)
+    fo, _ = O.lr_features(s.u, a_ref, s.v, s.L)
+    counts, pes = set(), []
+    for lanes in (1, 2, 4, 8, 16, 32):
+        print(f"[case] lanes {lanes}", flush=True)
+        monkeypatch.setenv("NBX_LANES", str(lanes))
+        ctx, _ = lj_pair(500, s.L, prec=_lib.PREC_MIXED)
+        assert ctx.stats()["lanes"] == lanes
+        assert relerr(_mixed_accel(ctx, s.u, np.zeros_like(s.u)), a_ref) <= MIXED_TOL, lanes
+        ctx.set_state(s.u, s.v)
+        counts.add(ctx.count_pairs())
+        f, _ = ctx.features()
+        assert np.abs(f[:, :32] - fo[:, :32]).max() <= 1e-5 * np.abs(fo[:, :32]).max(), lanes
+        pes.append(ctx.learn_energy()[1])
+        cw, ow, w = water_ctx(4, O.TERM_RF, prec=_lib.PREC_MIXED)
+        assert cw.stats()["lanes"] == lanes
+        aw = _mixed_accel(cw, w.u, np.zeros_like(w.u))
+        assert relerr(aw, ow.rhs(w.u)) <= MIXED_TOL, lanes
+    assert len(counts) == 1
+    assert max(abs(p - pes[0]) for p in pes) <= 1e-6 * abs(pes[0])
+
+
 def test_kernel_timing_over_a_run(c1):
     # one timed launch per step (graphs off while timing), and the same trajectory bit for
     # bit as the untimed (graph-replayed) run
