@@ -354,6 +354,33 @@ def test_every_lane_count_gives_the_same_physics(c1, monkeypatch):
     assert max(abs(p - pes[0]) for p in pes) <= 1e-6 * abs(pes[0])
 
 
+def test_list_overflow_replays_the_chunk():
+    # an imploding gas: list capacity is sized for the initial density and is exceeded
+    # inside a run; the chunk is restored from its device backup and replayed with larger
+    # lists.  The run must follow the FP64 path (FP32 pair forces aside) and the lists
+    # must have grown on the way.
+    s = gen.lj_liquid(6, rho=0.25, T=0.1, seed=5, jitter=0.05)
+    centre = 0.5 * s.L
+    v = s.v - 2.0 * (s.u - centre) / centre
+    mixed, _ = lj_pair(864, s.L, prec=_lib.PREC_MIXED)
+    mixed.set_state(s.u, v)
+    mixed.vv_run(0.002, 1)
+    slots0 = mixed.stats()["list_slots"]
+    mixed.set_state(s.u, v)
+    mixed.vv_run(0.002, 500)
+    slots1 = mixed.stats()["list_slots"]
+    ref, _ = lj_pair(864, s.L)
+    ref.set_state(s.u, v)
+    ref.vv_run(0.002, 500)
+    um, _ = mixed.get_state()
+    ur, _ = ref.get_state()
+    dev = float(np.abs(um - ur).max())
+    print(f"[case] implosion: list slots {slots0} -> {slots1}, max |du| vs FP64 {dev:.2e}",
+          flush=True)
+    assert slots1 > slots0
+    assert dev < 1e-5
+
+
 def test_kernel_timing_over_a_run(c1):
     # one timed launch per step (graphs off while timing), and the same trajectory bit for
     # bit as the untimed (graph-replayed) run
