@@ -253,7 +253,8 @@ int nbx_batch_run(nbx_batch* b, double dt, int64_t nsteps, uint64_t seed,
 /* per-replica kinetic and potential energy (reference conventions) and in-cutoff
  * ordered pair counts of the current state; each array holds nrep values (or NULL) */
 int nbx_batch_energies(nbx_batch* b, double* ke, double* pe, int64_t* pairs);
-/* out[0] list builds summed over replicas, [1] steps taken */
+/* out[0] list builds summed over replicas, [1] steps taken, [2] list slots per atom
+ * (nout <= 3) */
 int nbx_batch_stats(nbx_batch* b, int64_t* out, int32_t nout);
 
 /* ---------------------------------------------------------------------------------

@@ -469,11 +469,11 @@ extern "C" int nbx_batch_energies(nbx_batch* b, double* ke, double* pe, int64_t*
 }
 
 extern "C" int nbx_batch_stats(nbx_batch* b, int64_t* out, int32_t nout) {
-    if (!b || !out || nout < 0 || nout > 2) return NBX_ERR_INVALID;
+    if (!b || !out || nout < 0 || nout > 3) return NBX_ERR_INVALID;
     int h[8] = {0};
     BHIP(b, hipMemcpyAsync(h, b->d_flags, sizeof(h), hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
-    const int64_t v[2] = {h[1], b->steps_taken};
+    const int64_t v[3] = {h[1], b->steps_taken, b->cap};
     for (int k = 0; k < nout; ++k) out[k] = v[k];
     return NBX_OK;
 }

@@ -462,9 +462,9 @@ class Batch:
         return ke, pe, pairs
 
     def stats(self):
-        out = (C.c_int64 * 2)()
-        self._check(self._lib.nbx_batch_stats(self._h, out, 2))
-        return dict(list_builds=out[0], steps_taken=out[1])
+        out = (C.c_int64 * 3)()
+        self._check(self._lib.nbx_batch_stats(self._h, out, 3))
+        return dict(list_builds=out[0], steps_taken=out[1], list_slots=out[2])
 
 
 def device_count() -> int:

@@ -153,3 +153,40 @@ def test_batch_autotune_keeps_state_and_is_reproducible(nbx):
     ub, vb = b.get_state()
     uc, vc = c.get_state()
     assert np.array_equal(ub, uc) and np.array_equal(vb, vc)
+
+
+def test_batch_list_overflow_replays_the_call(nbx):
+    # imploding dilute replicas: the per-atom list capacity sized for the initial density
+    # is exceeded inside the launch; the call is restored from its device backup and
+    # replayed with larger lists.  Every replica must follow the single-system path.
+    print("[case] batch implosion", flush=True)
+    systems = [gen.lj_liquid(c, rho=0.25, T=0.1, seed=5 + c, jitter=0.05) for c in (5, 6)]
+    vs = []
+    for s in systems:
+        centre = 0.5 * s.L
+        vs.append(s.v - 2.0 * (s.u - centre) / centre)
+    natoms = [s.u.shape[0] for s in systems]
+    b = _lib.Batch(natoms, [s.L for s in systems], np.concatenate([s.masses for s in systems]),
+                   1.0, 1.0, 2.5, 1.0)
+    b.set_state(np.concatenate([s.u for s in systems]), np.concatenate(vs))
+    slots0 = b.stats()["list_slots"]
+    b.run(0.002, 500)
+    slots1 = b.stats()["list_slots"]
+    Ub, _ = b.get_state()
+    o, devs = 0, []
+    for s, v in zip(systems, vs):
+        n = s.u.shape[0]
+        ctx = _lib.Context(n, _lib.SYS_POTENTIAL, s.masses)
+        ctx.set_boundary(_lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+        ctx.add_lennard_jones(1, 1, 2.5)
+        ctx.set_kb(1.0)
+        ctx.set_precision(_lib.PREC_MIXED)
+        ctx.set_state(s.u, v)
+        ctx.vv_run(0.002, 500)
+        u, _ = ctx.get_state()
+        devs.append(float(np.abs(Ub[o:o + n] - u).max()))
+        o += n
+    print(f"[case] batch implosion: list slots {slots0} -> {slots1}, max |du| {max(devs):.2e}",
+          flush=True)
+    assert slots1 > slots0
+    assert max(devs) < 2e-5
