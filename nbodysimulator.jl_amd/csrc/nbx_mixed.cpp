@@ -266,7 +266,7 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
 // block of that kernel: mx_epilogue)
 void after_drift(nbx_ctx* c, const mx_plan& p, int par_out) {
     hipStream_t s = c->stream;
-    if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3)
+    if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell > 3)
         mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
                       bin_atoms(c, par_out));
 }
@@ -306,9 +306,16 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
 
 void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
     if (p.kind == PLAN_GRAVITY || !p.nb.mode_list) return;
-    mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
-                    bin_count(c, par), bin_atoms(c, par),
-                    c->nl.ncell >= 3 ? bin_count(c, par ^ 1) : nullptr);
+    // 3 cells per side: the 27 neighbour cells are the whole box, so any partition of
+    // the atoms into home cells finds every neighbour -- the bins made at set-up serve
+    // every build and no step bins
+    if (c->nl.ncell == 3)
+        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
+                        bin_count(c, 0), bin_atoms(c, 0), nullptr);
+    else
+        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
+                        bin_count(c, par), bin_atoms(c, par),
+                        c->nl.ncell > 3 ? bin_count(c, par ^ 1) : nullptr);
 }
 
 struct backup4 {
@@ -502,7 +509,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         for (int attempt = 0;; ++attempt) {
             mx_launch_step_advance(s, c->d_step, step0, 1);
             // start of step 1 of the chunk: X[0] -> X[1], bins B[1], request req[1]
-            if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3) {
+            if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell > 3) {
                 const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
                 mx_launch_fill_int(s, bin_count(c, 1), (int64_t)ncells, 0);
             }

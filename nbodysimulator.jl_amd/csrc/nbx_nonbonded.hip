@@ -15,7 +15,9 @@
 //                    rebuilt when an atom moved more than skin/2 since the last build:
 //                    the drift sets a request flag, and the binning launch after it and
 //                    the build launch of the next step read the flag and exit at once
-//                    when it is clear
+//                    when it is clear.  With 3 cells per side the 27 neighbour cells
+//                    are the whole box: the bins made at set-up serve every build and
+//                    no step bins
 //
 // Pair law (per ordered pair, FP32): r2 = |r|^2; LJ (2 s^12 - s^6)/r2 * r with
 // s^2 = sigma^2/r2; Coulomb q_j / r^3 * r; RF q_j (1/r^3 - 2 k_rf) * r.  Per-atom
