@@ -14,6 +14,7 @@
 // flag, [9] temperature, [10] Nose-Hoover zeta, [11] its rate xi, [12] pair count,
 // [13..14] list energies, [15] scratch rate for state-free RHS calls
 enum { NBX_SCAL_T = 0, NBX_SCAL_ZETA = 10, NBX_SCAL_XI = 11, NBX_SCAL_XI_SCRATCH = 15 };
+enum { NBX_BINS_STEP = 0, NBX_BINS_FIXED = 1, NBX_BINS_BUILD = 2, NBX_FLAG_BINSET = 5 };
 
 enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4, NBX_TERM_MAG = 5 };
 
@@ -29,16 +30,24 @@ struct nbx_nlist {
     int cap = 0;              // neighbour slots per atom (multiple of lanes)
     int ncell = 0;            // cells per side (0: all-pairs neighbour search)
     int ccap = 0;             // atoms per cell bound
+    // how bins follow the atoms: NBX_BINS_STEP -- every step bins (gated on a rebuild
+    // request) and the build scans the 27 neighbour cells; NBX_BINS_FIXED -- 3 cells
+    // per side, whose 27 neighbour cells are the whole box, so the set-up bins serve
+    // every build; NBX_BINS_BUILD -- half-size cells (>= (list radius + 1.5 skin) / 2),
+    // binned by each build for the next one, which scans 5 x 5 x 5 cells
+    int bin_mode = 0;
     double rlist = 0.0;       // list radius = cutoff + skin
     double cutoff = 0.0;      // largest interaction cutoff
     double skin = 0.0;        // skin in use (rebuild when an atom moved skin/2)
     int* d_nbr = nullptr;     // neighbour slots, interleaved by lanes (see nbx_nonbonded.hip)
     int* d_nnb = nullptr;     // neighbour counts
-    int* d_cell_count = nullptr;  // 2 x ncells (bins of even / odd steps)
-    int* d_cell_atoms = nullptr;  // 2 x ncells x ccap
+    int* d_cell_count = nullptr;  // 3 x ncells (NBX_BINS_STEP: sets of even / odd steps;
+                                  // NBX_BINS_BUILD: three sets rotating at each build)
+    int* d_cell_atoms = nullptr;  // 3 x ncells x ccap
     double* d_xref = nullptr; // unwrapped positions at the last build (SoA)
     int* d_flags = nullptr;   // [0] / [3] rebuild request for even / odd steps,
-                              // [1] overflow, [2] builds done, [4] reduction counter
+                              // [1] overflow, [2] builds done, [4] reduction counter,
+                              // [5] current bin set, [6] build finish counter
     size_t cap_bytes = 0;
 };
 

@@ -578,7 +578,9 @@ __global__ void __launch_bounds__(256) k_lr_drift(const double* __restrict__ par
         make_double4(mx_wrap(p3[0], io.L), mx_wrap(p3[1], io.L), mx_wrap(p3[2], io.L),
                      io.qs != nullptr ? io.qs[i] : 0.0);
     const double ex = p3[0] - io.xref[i], ey = p3[1] - io.xref[n + i], ez = p3[2] - io.xref[2 * n + i];
-    if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
+    const double e2 = ex * ex + ey * ey + ez * ez;
+    if (e2 > io.half_skin2) io.req_out[0] = 1;
+    if (e2 > io.guard2) io.flags[1] = 1;   // past what the bins of the last build cover
 }
 
 void lr_launch_drift(hipStream_t s, const double* part, int nparts, const float* dlab, double D,

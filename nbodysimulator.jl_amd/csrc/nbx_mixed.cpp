@@ -146,9 +146,25 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     cap = (int)std::min<int64_t>(cap, std::max<int64_t>(n, 1));
     nl.cap = round_up(std::max(cap, 1), lanes);
     int nc = (int)std::floor(L / nl.rlist);
-    if (nc < 3) nc = 0;
+    int mode = NBX_BINS_STEP;
+    if (nc < 3) {
+        nc = 0;
+    } else if (nc == 3) {
+        mode = NBX_BINS_FIXED;
+    } else {
+        // half-size cells binned by each build: with every atom moved at most 3/4 skin
+        // since (the drift flags more), 2 cells reach list radius + 1.5 skin.  The build
+        // gives every cell a workgroup, so this pays only with well-filled cells (water:
+        // ~57 atoms per half cell; a Lennard-Jones liquid: ~4, which keeps per-step bins)
+        const int nh = (int)std::floor(L / (0.5 * (nl.rlist + 1.5 * nl.skin)));
+        if (nh >= 5 && (double)n / ((double)nh * nh * nh) >= 16.0) {
+            nc = nh;
+            mode = NBX_BINS_BUILD;
+        }
+    }
     if (nc > 128) nc = 128;
     nl.ncell = nc;
+    nl.bin_mode = mode;
     const double per_cell = nc ? (double)n / ((double)nc * nc * nc) : 0.0;
     nl.ccap = nc ? (int)std::ceil(grow * (2.0 * per_cell + 16.0)) : 0;
     if (nl.ccap > 1024)
@@ -156,14 +172,14 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     const size_t ncells = (size_t)nc * nc * nc;
     NBX_HIP(c, hipMalloc((void**)&nl.d_nbr, sizeof(int) * (size_t)nl.cap * (size_t)n));
     NBX_HIP(c, hipMalloc((void**)&nl.d_nnb, sizeof(int) * (size_t)n));
-    NBX_HIP(c, hipMalloc((void**)&nl.d_cell_count, sizeof(int) * 2 * std::max<size_t>(ncells, 1)));
+    NBX_HIP(c, hipMalloc((void**)&nl.d_cell_count, sizeof(int) * 3 * std::max<size_t>(ncells, 1)));
     NBX_HIP(c, hipMalloc((void**)&nl.d_cell_atoms,
-                         sizeof(int) * 2 * std::max<size_t>(ncells * (size_t)nl.ccap, 1)));
+                         sizeof(int) * 3 * std::max<size_t>(ncells * (size_t)nl.ccap, 1)));
     NBX_HIP(c, hipMalloc((void**)&nl.d_xref, sizeof(double) * 3 * (size_t)n));
     int rc = alloc_flags(c);
     if (rc) return rc;
     NBX_HIP(c, hipMemsetAsync(nl.d_nnb, 0, sizeof(int) * (size_t)n, c->stream));
-    NBX_HIP(c, hipMemsetAsync(nl.d_cell_count, 0, sizeof(int) * 2 * std::max<size_t>(ncells, 1),
+    NBX_HIP(c, hipMemsetAsync(nl.d_cell_count, 0, sizeof(int) * 3 * std::max<size_t>(ncells, 1),
                               c->stream));
     nl.cap_bytes = sizeof(int) * (size_t)nl.cap * (size_t)n;
     nl.ready = true;
@@ -247,6 +263,8 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
     if (list) {
         io.xref = c->nl.d_xref;
         io.half_skin2 = 0.25 * c->nl.skin * c->nl.skin;
+        io.guard2 = c->nl.bin_mode == NBX_BINS_BUILD ? 0.5625 * c->nl.skin * c->nl.skin
+                                                      : INFINITY;
         io.req_out = req_flag(c, par ^ 1);
         io.L = c->bc.b[0];
     }
@@ -266,7 +284,8 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
 // block of that kernel: mx_epilogue)
 void after_drift(nbx_ctx* c, const mx_plan& p, int par_out) {
     hipStream_t s = c->stream;
-    if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell > 3)
+    if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3 &&
+        c->nl.bin_mode == NBX_BINS_STEP)
         mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
                       bin_atoms(c, par_out));
 }
@@ -309,13 +328,16 @@ void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
     // 3 cells per side: the 27 neighbour cells are the whole box, so any partition of
     // the atoms into home cells finds every neighbour -- the bins made at set-up serve
     // every build and no step bins
-    if (c->nl.ncell == 3)
+    if (c->nl.bin_mode == NBX_BINS_FIXED)
         mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
                         bin_count(c, 0), bin_atoms(c, 0), nullptr);
+    else if (c->nl.bin_mode == NBX_BINS_BUILD)
+        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
+                        c->nl.d_cell_count, c->nl.d_cell_atoms, nullptr);
     else
         mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
                         bin_count(c, par), bin_atoms(c, par),
-                        c->nl.ncell > 3 ? bin_count(c, par ^ 1) : nullptr);
+                        c->nl.ncell >= 3 ? bin_count(c, par ^ 1) : nullptr);
 }
 
 struct backup4 {
@@ -365,9 +387,10 @@ static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
     const int pk = pos_kind(p);
     if (pk == 0) return;
     const bool list = pk == 1 && c->nl.ncell >= 3 && bins;
-    if (list) {
+    if (list) {   // fresh bins in set 0, the other sets empty, rotation at set 0
         const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
-        mx_launch_fill_int(c->stream, c->nl.d_cell_count, (int64_t)(2 * ncells), 0);
+        mx_launch_fill_int(c->stream, c->nl.d_cell_count, (int64_t)(3 * ncells), 0);
+        mx_launch_fill_int(c->stream, c->nl.d_flags + NBX_FLAG_BINSET, 2, 0);
     }
     mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, list ? bin_count(c, 0) : nullptr,
                         list ? bin_atoms(c, 0) : nullptr,
@@ -509,7 +532,8 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         for (int attempt = 0;; ++attempt) {
             mx_launch_step_advance(s, c->d_step, step0, 1);
             // start of step 1 of the chunk: X[0] -> X[1], bins B[1], request req[1]
-            if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell > 3) {
+            if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3 &&
+                c->nl.bin_mode == NBX_BINS_STEP) {
                 const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
                 mx_launch_fill_int(s, bin_count(c, 1), (int64_t)ncells, 0);
             }

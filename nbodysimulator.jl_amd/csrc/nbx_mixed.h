@@ -68,7 +68,9 @@ struct mx_io {
     int do_drift;         // 0: finish the step only (no x_{k+1})
     // Verlet-list bookkeeping (list mode)
     const double* xref;   // positions at the last build
-    double half_skin2;
+    double half_skin2;    // (skin/2)^2: rebuild request
+    double guard2;        // NBX_BINS_BUILD: (3 skin/4)^2, past which the bins of the last
+                          // build may miss pairs (flagged as an overflow)
     int* req_out;         // rebuild request for step k+1
     int* flags;           // [1] overflow
     double L;             // cubic box side (wrapped copy)
@@ -165,7 +167,9 @@ __device__ __forceinline__ void mx_store_position(int64_t i, double px, double p
     }
     if (io.xref != nullptr) {
         const double ex = px - A.xr[0], ey = py - A.xr[1], ez = pz - A.xr[2];
-        if (ex * ex + ey * ey + ez * ez > io.half_skin2) io.req_out[0] = 1;
+        const double e2 = ex * ex + ey * ey + ez * ez;
+        if (e2 > io.half_skin2) io.req_out[0] = 1;
+        if (e2 > io.guard2) io.flags[1] = 1;
     }
 }
 
@@ -304,10 +308,11 @@ void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_pa
 // start of step kp.step + 1 from (x_in, v, a): drift of every atom
 void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
                      const mx_io& io);
-// Verlet list from bins B (gated by req), clearing bins `clear_count` first
+// Verlet list (gated by req) from bins B, clearing bins `clear_count` first
+// (NBX_BINS_BUILD: B is the base of the three rotating sets, which the build advances)
 void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
-                     const double* x, const int* req, const int* bin_count,
-                     const int* bin_atoms, int* clear_count);
+                     const double* x, const int* req, int* bin_count, int* bin_atoms,
+                     int* clear_count);
 // wrapped copy of x into pos, binned into bins; sets *req
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
                          int pos_kind, int* bin_count, int* bin_atoms, int* req);

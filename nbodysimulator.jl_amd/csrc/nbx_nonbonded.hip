@@ -427,20 +427,22 @@ __device__ __forceinline__ int build_segment(const int* seg, int nseg, int t) {
     return lo;
 }
 
+#define BUILD_STENCIL 125
+
 template <bool WATER>
 __global__ void __launch_bounds__(BUILD_THREADS)
 k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int n, int nc,
-              float Lf, float invLf, const int* __restrict__ count,
-              const int* __restrict__ atoms, int ccap, float rlist2f, int G, int cap,
+              float Lf, float invLf, int* __restrict__ bin_count, int* __restrict__ bin_atoms,
+              int ccap, double inv_cs, float rlist2f, int G, int cap,
               int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
               int* __restrict__ flags, const int* __restrict__ req,
-              int* __restrict__ clear_count, int parts) {
+              int* __restrict__ clear_count, int rotate, int parts) {
     __shared__ int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
     __shared__ int sord[BUILD_SLOTS];
     __shared__ unsigned char sseg[BUILD_SLOTS];
-    __shared__ int seg[28], segcell[27];
-    __shared__ int ncnt[27], ncell[27];
+    __shared__ int seg[BUILD_STENCIL + 1], segcell[BUILD_STENCIL];
+    __shared__ int ncnt[BUILD_STENCIL], ncell[BUILD_STENCIL];
     __shared__ int okey[BUILD_OWN], ocnt[BUILD_OWN];
     __shared__ float4 opos[BUILD_OWN];
     const int tid = threadIdx.x;
@@ -451,10 +453,22 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     const int cid = blockIdx.x / parts, part = blockIdx.x % parts;
     if (clear_count != nullptr && tid == 0 && part == 0) clear_count[cid] = 0;
     if (req[0] == 0) return;
+    const int ncells = nc * nc * nc;
+    // rotate: the bins of the previous build (set `bset`) are read, the bins of the
+    // current positions go to the next set, and the set after it is cleared
+    int* state = flags + NBX_FLAG_BINSET;
+    const int bset = rotate ? state[0] : 0;
+    const int* count = bin_count + (size_t)bset * ncells;
+    const int* atoms = bin_atoms + (size_t)bset * ncells * ccap;
+    int* next_count = bin_count + (size_t)((bset + 1) % 3) * ncells;
+    int* next_atoms = bin_atoms + (size_t)((bset + 1) % 3) * ncells * ccap;
+    if (rotate && tid == 0 && part == 0) bin_count[(size_t)((bset + 2) % 3) * ncells + cid] = 0;
     if (blockIdx.x == 0 && tid == 0) atomicAdd(&flags[2], 1);
     const int cz = cid / (nc * nc), cy = (cid / nc) % nc, cx = cid % nc;
-    if (tid < 27) {  // the 27 neighbour cells (fixed order) and their counts, in parallel
-        const int oz = tid / 9 - 1, oy = (tid / 3) % 3 - 1, ox = tid % 3 - 1;
+    const int nst = rotate ? 125 : 27;
+    if (tid < nst) {  // the stencil cells (fixed order) and their counts, in parallel
+        const int w = rotate ? 5 : 3, h = rotate ? 2 : 1;
+        const int oz = tid / (w * w) - h, oy = (tid / w) % w - h, ox = tid % w - h;
         const int nid = (((cz + oz + nc) % nc) * nc + (cy + oy + nc) % nc) * nc + (cx + ox + nc) % nc;
         const int m2 = count[nid];
         ncell[tid] = nid;
@@ -464,7 +478,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     m = m < ccap ? m : ccap;
     const int per = (m + parts - 1) / parts;
     const int hb = part * per, he = (hb + per) < m ? (hb + per) : m;
-    if (hb >= he) return;   // uniform over the block, before its first barrier
+    if (hb < he) {   // uniform over the block
     __syncthreads();
     const size_t stride = (size_t)n * G;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -476,12 +490,19 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             okey[t] = i;
             opos[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
             ocnt[t] = 0;
+            if (rotate) {   // the bin of the atom's current position, for the next build
+                const int c2 = (mx_cell_coord(p.z, inv_cs, nc) * nc +
+                                mx_cell_coord(p.y, inv_cs, nc)) * nc + mx_cell_coord(p.x, inv_cs, nc);
+                const int slot = atomicAdd(&next_count[c2], 1);
+                if (slot < ccap) next_atoms[(size_t)c2 * ccap + slot] = i;
+                else flags[1] = 1;
+            }
         }
         int nb = 0;
-        while (nb < 27) {
-            // a batch of consecutive neighbour cells that fits in LDS
+        while (nb < nst) {
+            // a batch of consecutive stencil cells that fits in LDS
             int M = 0, nseg = 0;
-            while (nb < 27) {
+            while (nb < nst) {
                 const int m2 = ncnt[nb];
                 if (M + m2 > BUILD_SLOTS && nseg > 0) break;
                 if (tid == 0) { seg[nseg] = M; segcell[nseg] = ncell[nb]; }
@@ -549,6 +570,16 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
         }
         __syncthreads();
     }
+    }
+    // rotate: the last block to finish advances the set (every block read it at its start)
+    if (rotate && tid == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(reinterpret_cast<unsigned*>(&state[1]), 1u);
+        if (done == gridDim.x - 1) {
+            state[1] = 0;
+            state[0] = (bset + 1) % 3;
+        }
+    }
 }
 
 // small boxes (fewer than 3 cells per side): every atom scans all atoms
@@ -586,8 +617,8 @@ __global__ void k_build_allpairs(const double4* __restrict__ pos, const double* 
 }
 
 void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
-                     const double* x, const int* req, const int* bin_count,
-                     const int* bin_atoms, int* clear_count) {
+                     const double* x, const int* req, int* bin_count, int* bin_atoms,
+                     int* clear_count) {
     const nbx_nlist& nl = c->nl;
     const int n = (int)c->n;
     if (n <= 0) return;
@@ -597,6 +628,7 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
     if (nc >= 3) {
         const int ncells = nc * nc * nc;
         const float r2f = (float)r2 * 1.0001f;
+        const int rotate = nl.bin_mode == NBX_BINS_BUILD ? 1 : 0;
         // at least ~512 blocks (two per CU) when the cells are few: up to 8 per cell
         int parts = (512 + ncells - 1) / ncells;
         parts = parts < 1 ? 1 : (parts > 8 ? 8 : parts);
@@ -604,13 +636,15 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
         if (water)
             hipLaunchKernelGGL(k_build_cells<true>, grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
-                               bin_count, bin_atoms, nl.ccap, r2f, nl.lanes, nl.cap, nl.d_nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, parts);
+                               bin_count, bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes,
+                               nl.cap, nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req,
+                               clear_count, rotate, parts);
         else
             hipLaunchKernelGGL(k_build_cells<false>, grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
-                               bin_count, bin_atoms, nl.ccap, r2f, nl.lanes, nl.cap, nl.d_nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, parts);
+                               bin_count, bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes,
+                               nl.cap, nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req,
+                               clear_count, rotate, parts);
     } else {
         if (water)
             hipLaunchKernelGGL(k_build_allpairs<true>, dim3(nbx_blocks(n, 128)), dim3(128), 0, s,

@@ -118,6 +118,25 @@ def test_water_4096(nbx):
     am = mixed_accel(out[1], w.u)
     assert relerr(am, ref) <= 2e-6
     assert momentum_ratio(am, w.masses) <= 1e-5
+    # list builds on the bins of the previous build (half-size cells of at least
+    # (list radius + 1.5 skin) / 2, 5 x 5 x 5 stencil): a small skin rebuilds often, and
+    # the trajectory must follow the FP64 path's (a pair missed by a stale bin would move
+    # atoms by ~1e-3 nm)
+    skin = 0.06
+    for c in out:
+        c.set_state(w.u, w.v)
+    out[1].set_skin(skin)
+    out[1].vv_run(0.0005, 200)
+    out[0].vv_run(0.0005, 200)
+    st = out[1].stats()
+    um, _ = out[1].get_state()
+    ur, _ = out[0].get_state()
+    dev = float(np.abs(um - ur).max())
+    print(f"[case] water rebuilds: {st['list_builds']} builds, {st['cells_per_side']} cells per "
+          f"side, max |du| vs FP64 {dev:.2e} nm", flush=True)
+    assert st["cells_per_side"] == int(w.L // (0.5 * (Rc + skin + 1.5 * skin)))
+    assert st["list_builds"] >= 5
+    assert dev < 1e-5
 
 
 def test_batch_512_replicas(nbx):
