@@ -11,13 +11,15 @@
 //   nbr            : Verlet list, interleaved for G lanes per atom: slot k of atom i is
 //                    at (k / G) * (n * G) + i * G + (k % G), so lane l of atom i reads
 //                    consecutive words with the other lanes of the wavefront
-//   B[2]           : fixed-capacity cell bins (count + ccap slots per cell); a list is
+//   B[3]           : fixed-capacity cell bins (count + ccap slots per cell); a list is
 //                    rebuilt when an atom moved more than skin/2 since the last build:
-//                    the drift sets a request flag, and the binning launch after it and
-//                    the build launch of the next step read the flag and exit at once
-//                    when it is clear.  With 3 cells per side the 27 neighbour cells
-//                    are the whole box: the bins made at set-up serve every build and
-//                    no step bins
+//                    the drift sets a request flag, and the build launch of the next
+//                    step (and, with per-step bins, the binning launch after the drift)
+//                    read the flag and exit at once when it is clear.  Bin modes: per-
+//                    step bins (two sets by step parity); fixed bins (3 cells per side:
+//                    the 27 neighbour cells are the whole box, so the set-up bins serve
+//                    every build); bins made by each build for the next one (half-size
+//                    cells, three rotating sets, 5 x 5 x 5 stencil)
 //
 // Pair law (per ordered pair, FP32): r2 = |r|^2; LJ (2 s^12 - s^6)/r2 * r with
 // s^2 = sigma^2/r2; Coulomb q_j / r^3 * r; RF q_j (1/r^3 - 2 k_rf) * r.  Per-atom
@@ -401,8 +403,10 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
 
 // ---------------------------------------------------------------------------
 // Verlet list build (gated by *req): one block of 4 wavefronts per cell.
-//   1. the cell's own atoms (<= 256 per pass) are staged in LDS;
-//   2. the atoms of the 27 neighbour cells (fixed order; in batches of <= 1024) are
+//   1. the cell's own atoms (<= 256 per pass) are staged in LDS (rotate: and binned by
+//      their current positions into the next bin set);
+//   2. the atoms of the stencil cells -- the 27 neighbour cells, or with rotate the 125
+//      cells within two half-size cells (fixed order; in batches of <= 1024) -- are
 //      staged in LDS -- indices, then FP32 positions, every load independent -- and
 //      rank-sorted by index within each cell, so every list comes out in a reproducible
 //      order (cell order, then index order);
@@ -410,7 +414,12 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
 //      a time and append the hits in candidate order (ballot + prefix count).
 //   The squared list radius carries a 1e-4 relative margin so that the FP32 test never
 //   drops a pair that the FP64 distance puts inside the list radius.
-// Every block also clears its cell in the other bin set (used by the next step).
+// Per-step bins: every block also clears its cell in the other bin set (used by the next
+// step).  Rotate (bins made by the builds): the build reads set b % 3 (the bins of the
+// previous build; every atom has since moved at most 3/4 skin, which the drift enforces,
+// and two half cells reach list radius + 1.5 skin), fills set (b + 1) % 3 (cleared by the
+// previous build) and clears set (b + 2) % 3; the last block to finish (device-scope
+// counter) advances b, which every block read at its start.
 // ---------------------------------------------------------------------------
 
 #define BUILD_THREADS 256
