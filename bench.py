@@ -203,8 +203,9 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
         ctx.set_state(s.u, s.v)
         tuned = None
         if mode[0] == "mixed":
-            # launch shape (lanes per atom, list skin) picked by the autotuner
-            tuned = ctx.autotune(0.005, 200)
+            # launch shape (lanes per atom, list skin) picked by the autotuner; 1,000
+            # trial steps per candidate so that small skins pay their rebuilds in the trial
+            tuned = ctx.autotune(0.005, 1000)
         ke0, pe0, _ = ctx.energies()
         nst = steps if mode[0] == "mixed" else max(steps // 10, 10)
         ctx.vv_run(0.005, 20)
@@ -281,7 +282,7 @@ def bench_water(args, lib, gen):
     Rc = 0.9
     dt, steps = 0.0005, 2000
     ctx = water_context(lib, gen, w, Rc, True)
-    tuned = ctx.autotune(dt, 100)
+    tuned = ctx.autotune(dt, 400)
     ctx.vv_run(dt, 20)
     t0 = time.perf_counter()
     ctx.vv_run(dt, steps)
@@ -415,7 +416,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
 
     def run(model, stride):
         ctx, _ = ctx_for(4242)
-        ctx.autotune(dt, 200)     # the launch shape of the plain reference run below
+        ctx.autotune(dt, 1000)    # the launch shape of the plain reference run below
         t0 = time.perf_counter()
         rep = ctx.learned_run(model, dt, stride, nsteps_dt // stride, drift_tol=1e-4,
                               check_every=25)
@@ -427,7 +428,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
 
     # plain velocity Verlet at dt: the reference point and the yardstick
     plain, s = ctx_for(4242)
-    plain.autotune(dt, 200)
+    plain.autotune(dt, 1000)
     ke0, pe0 = plain.learn_energy()
     plain.vv_run(dt, 20)
     plain.set_state(s.u, s.v)
