@@ -158,6 +158,11 @@ int nbx_em_run(nbx_ctx* ctx, double dt, int64_t nsteps, uint64_t seed, int64_t s
 /* Steps taken by all nbx_vv_run / nbx_em_run calls since the last set_state
  * (solution.stats.naccept). */
 int64_t nbx_steps_taken(const nbx_ctx* ctx);
+/* Nose-Hoover thermostat state of the current step: zeta and its rate xi, which the
+ * reference keeps in the extra state column, u[3n+1] and v[3n+1]
+ * (gather_bodies_initial_coordinates src/nbody_to_ode.jl:1-20, nosehoover_acceleration!
+ * src/thermostats.jl:120-126).  Both are 0 after nbx_set_state and without Nose-Hoover. */
+int nbx_get_thermostat_state(const nbx_ctx* ctx, double* zeta, double* xi);
 
 /* Energies of the current state, in FP64 on the device:
  *   ke = kinetic_energy (src/nbody_simulation_result.jl:209-218)

@@ -79,6 +79,7 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     c->h_ms.assign(masses, masses + n);
     c->equal_masses = n > 0 && std::all_of(masses, masses + n,
                                            [&](double m) { return m == masses[0]; });
+    if (n > 0) c->mass_wscale = 1.0 / *std::max_element(masses, masses + n);
     if (charges) {
         c->h_qs.assign(charges, charges + n);
         c->has_q = true;
@@ -365,6 +366,19 @@ extern "C" int nbx_get_state(const nbx_ctx* c, double* u, double* v) {
 }
 
 extern "C" int64_t nbx_steps_taken(const nbx_ctx* c) { return c ? c->steps_taken : -1; }
+
+// Nose-Hoover extended coordinate and its rate (the reference's u[3n+1] and v[3n+1])
+extern "C" int nbx_get_thermostat_state(const nbx_ctx* c, double* zeta, double* xi) {
+    if (!c) return NBX_ERR_INVALID;
+    nbx_ctx* m = const_cast<nbx_ctx*>(c);
+    double h[2] = {0.0, 0.0};
+    NBX_HIP(m, hipMemcpyAsync(h, c->d_scal + NBX_SCAL_ZETA, sizeof(h), hipMemcpyDeviceToHost,
+                              c->stream));
+    NBX_HIP(m, hipStreamSynchronize(c->stream));
+    if (zeta) *zeta = h[0];
+    if (xi) *xi = h[1];
+    return NBX_OK;
+}
 
 // ---------------------------------------------------------------------------
 // FP64 reference-order right-hand side

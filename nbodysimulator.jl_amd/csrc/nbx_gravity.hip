@@ -9,9 +9,12 @@
 // accumulator, and chunk partials are stored in FP64 and added by the kick kernel in
 // chunk order (deterministic).
 //   a_i = G * sum_j m_j (r_j - r_i) / |r_j - r_i|^3      (= -G m_j r_ij / |r_ij|^3)
-// The self term is removed by adding 1e-20 to r^2 inside the FMA chain (no extra
-// instruction): for j == i the displacement is exactly zero, so its contribution is
-// exactly zero; for distinct bodies the addition rounds away.
+// The self term is removed by adding 1e-24 to r^2 inside the FMA chain (no extra
+// instruction): for j == i the displacement is exactly zero and 1/r^3 = 1e36 is finite,
+// so the contribution is exactly zero.  The mass in .w is m / m_max <= 1 (k_grav_finish
+// multiplies by G m_max), so m_j / r^3 stays finite for any mass scale; for distinct
+// bodies the 1e-24 rounds away unless r^2 < 2^24 * 1e-24 (r < 4e-9 in the caller's
+// length unit).
 // When every body has the same mass m (EQM), the tiles sum (r_j - r_i) / |r_j - r_i|^3
 // and k_grav_finish multiplies by G m in FP64: one multiply fewer per pair (12 VALU
 // instructions and one v_rsq_f32 instead of 13 and one).
@@ -55,10 +58,9 @@ k_grav_tiled(const float4* __restrict__ pos, int n, int jchunk, double* __restri
 #pragma unroll
             for (int t = 0; t < TPT; ++t) {
                 const float dx = pj.x - xi[t], dy = pj.y - yi[t], dz = pj.z - zi[t];
-                // r^2 + 1e-20 inside the FMA chain: the self term (dx = dy = dz = 0) gets a
-                // finite 1/r^3 times a zero displacement; for any two distinct FP32
-                // positions r^2 >> 2^24 * 1e-20, so the sum rounds to r^2 exactly
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
+                // r^2 + 1e-24 inside the FMA chain: the self term (dx = dy = dz = 0) gets a
+                // finite 1/r^3 (1e36, times a mass factor <= 1) times a zero displacement
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-24f)));
                 const float rinv = __builtin_amdgcn_rsqf(r2);
                 const float s = EQM ? rinv * rinv * rinv : pj.w * (rinv * rinv * rinv);
                 sx[t] += dx * s;
@@ -83,7 +85,8 @@ k_grav_tiled(const float4* __restrict__ pos, int n, int jchunk, double* __restri
 }
 
 // a_i = gscale * sum over chunks (chunk order), then the end of the step and the drift;
-// gscale = G, or G m when the tiles left out the common mass m
+// gscale = G m_max (the tiles summed m_j / m_max), or G m when the tiles left out the
+// common mass m
 __global__ void __launch_bounds__(256)
 k_grav_finish(const double* __restrict__ part, int chunks, int n, double gscale,
               mx_kick_params kp, mx_io io) {
@@ -147,7 +150,7 @@ void mx_launch_gravity(hipStream_t s, const nbx_ctx* c, const void* pos, double 
     }
     if (ev1) (void)hipEventRecord(ev1, s);
     if (force_only_kernel) return;
-    const double scale = eqm ? G * c->h_ms[0] : G;
+    const double scale = eqm ? G * c->h_ms[0] : G / c->mass_wscale;
     hipLaunchKernelGGL(k_grav_finish, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, c->d_part,
                        chunks, n, scale, kp, io);
 }

@@ -121,6 +121,9 @@ struct nbx_ctx {
     int grav_tpt_set = 0;     // targets per thread of the gravity kernel (0: by size)
     int grav_chunks_set = 0;  // source chunks of the gravity kernel (0: by size)
     bool equal_masses = false;  // every body has mass h_ms[0] (gravity takes m out of the sum)
+    // 1 / (largest mass): the FP32 gravity positions carry m / m_max, so that the self
+    // term's finite 1/r^3 guard times the mass stays finite for any mass scale
+    double mass_wscale = 1.0;
     nbx_nlist nl;
     int64_t* d_step = nullptr;       // absolute step base read by replayed graphs
     hipGraphExec_t gexec = nullptr;  // captured block of mixed-path steps

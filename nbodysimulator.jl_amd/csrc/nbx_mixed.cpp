@@ -188,6 +188,12 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
 
 bool all_pairs_tiled(const mx_plan& p) { return p.kind == PLAN_GRAVITY || p.kind == PLAN_MAGNETIC; }
 
+// factor on the mass carried in the FP32 positions: gravity keeps m / m_max there (the
+// magnetic kernel divides by the plain mass)
+double mass_wscale(const nbx_ctx* c, const mx_plan& p) {
+    return p.kind == PLAN_GRAVITY ? c->mass_wscale : 1.0;
+}
+
 int pos_kind(const mx_plan& p) {
     if (all_pairs_tiled(p)) return 2;
     return p.nb.mode_list ? 1 : 0;
@@ -251,6 +257,7 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
     io.x_in = xbuf(c, par);
     io.x_out = xbuf(c, par ^ 1);
     io.pos_kind = pos_kind(p);
+    io.wscale = mass_wscale(c, p);
     io.pos_out = pbuf(c, par ^ 1);
     io.a = c->d_a;
     io.v = c->d_v;
@@ -392,7 +399,8 @@ static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
         mx_launch_fill_int(c->stream, c->nl.d_cell_count, (int64_t)(3 * ncells), 0);
         mx_launch_fill_int(c->stream, c->nl.d_flags + NBX_FLAG_BINSET, 2, 0);
     }
-    mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, list ? bin_count(c, 0) : nullptr,
+    mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, mass_wscale(c, p),
+                        list ? bin_count(c, 0) : nullptr,
                         list ? bin_atoms(c, 0) : nullptr,
                         (pk == 1 && bins) ? req_flag(c, 0) : nullptr);
 }
@@ -914,8 +922,8 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
     if ((rc = dalloc(c, bk, sizeof(double) * 3 * n3))) return rc;
     NBX_HIP(c, hipMemsetAsync(ffrag.p, 0, 2 * (size_t)ntiles * 3 * LR_FRAG_ELEMS, s));
     if ((rc = fresh_list(c, p))) return rc;
-    // energy-partials buffer sized for the largest lane count the lists may use
-    if ((rc = dalloc(c, epart, sizeof(double) * 2 * lr_energy_blocks(n, 16)))) return rc;
+    // energy-partials buffer sized for the largest lane count the lists may use (32)
+    if ((rc = dalloc(c, epart, sizeof(double) * 2 * lr_energy_blocks(n, 32)))) return rc;
     double ke0, pe0;
     if ((rc = list_energy(c, 0, lj, epart, &ke0, &pe0))) return rc;
     const double E0 = ke0 + pe0;

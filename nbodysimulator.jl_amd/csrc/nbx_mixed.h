@@ -58,8 +58,9 @@ struct mx_nh_params {
 struct mx_io {
     const double* x_in;   // x_k (SoA, unwrapped)
     double* x_out;        // x_{k+1}
-    void* pos_out;        // wrapped copy of x_{k+1}: double4 (+q) or float4 (+m)
+    void* pos_out;        // wrapped copy of x_{k+1}: double4 (+q) or float4 (+m wscale)
     int pos_kind;         // 0 none, 1 double4 wrapped, 2 float4
+    double wscale;        // pos_kind 2: factor on the mass in .w (gravity: 1 / m_max)
     double* a;            // a_k out
     double* v;            // v_k out
     double* vh;           // in: vh_k, out: vh_{k+1}
@@ -163,7 +164,8 @@ __device__ __forceinline__ void mx_store_position(int64_t i, double px, double p
         const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
         ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, A.q);
     } else if (io.pos_kind == 2) {
-        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz, (float)A.m);
+        ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz,
+                                               (float)(A.m * io.wscale));
     }
     if (io.xref != nullptr) {
         const double ex = px - A.xr[0], ey = py - A.xr[1], ez = pz - A.xr[2];
@@ -315,7 +317,7 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
                      int* clear_count);
 // wrapped copy of x into pos, binned into bins; sets *req
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
-                         int pos_kind, int* bin_count, int* bin_atoms, int* req);
+                         int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req);
 void mx_launch_temperature(hipStream_t s, const nbx_ctx* c, const double* v, int64_t Nc,
                            double* d_T);
 void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& L,

@@ -333,7 +333,7 @@ void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value) {
 
 // wrapped copy of x (after set_state), binned; requests a list build
 __global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, void* pos,
-                               double L, const double* __restrict__ ms,
+                               double L, double wscale, const double* __restrict__ ms,
                                const double* __restrict__ qs, int* __restrict__ bin_count,
                                int* __restrict__ bin_atoms, int ccap, int nc, double inv_cs,
                                int* flags, int* req) {
@@ -352,16 +352,16 @@ __global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, vo
             else flags[1] = 1;
         }
     } else if (kind == 2) {
-        ((float4*)pos)[i] = make_float4((float)px, (float)py, (float)pz, (float)ms[i]);
+        ((float4*)pos)[i] = make_float4((float)px, (float)py, (float)pz, (float)(ms[i] * wscale));
     }
 }
 
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
-                         int pos_kind, int* bin_count, int* bin_atoms, int* req) {
+                         int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req) {
     if (c->n <= 0) return;
     const int nc = c->nl.ncell;
     hipLaunchKernelGGL(k_mx_positions, dim3(nbx_blocks(c->n, 256)), dim3(256), 0, s, x,
-                       (int)c->n, pos_kind, pos, c->bc.b[0], c->d_ms,
+                       (int)c->n, pos_kind, pos, c->bc.b[0], wscale, c->d_ms,
                        c->has_q ? c->d_qs : nullptr, nc >= 3 ? bin_count : nullptr, bin_atoms,
                        c->nl.ccap, nc, nc > 0 ? (double)nc / c->bc.b[0] : 0.0, c->nl.d_flags, req);
 }

@@ -11,9 +11,13 @@ thermostats) are used unchanged.  Two drop-in points:
     `calculate_simulation` (src/nbody_simulation_result.jl:480-487) and returns a
     `SimulationResult` over an `ODESolution` of `ArrayPartition(v, u)` frames, so
     `get_position`, `temperature`, energies and `rdf` keep working;
-  * `get_accelerating_function(MI355XPotential(p), sim)` replaces the per-atom closure of
-    `p` (src/nbody_to_ode.jl:156-242): the first call of an RHS evaluation computes all
-    atoms on the GPU, later calls add the cached column.
+  * `MI355XPotential(p)` is a `PotentialParameters` that wraps a built-in parameter object
+    `p` and goes where `p` went, in `PotentialNBodySystem.potentials`
+    (src/nbody_system.jl:72-75): `gather_accelerations_for_potentials`
+    (src/nbody_to_ode.jl:87-99) then asks `get_accelerating_function(MI355XPotential(p),
+    sim)` for its closure, whose first call of an RHS evaluation (i == 1,
+    src/nbody_to_ode.jl:474-488) computes all atoms on the GPU and whose calls add the
+    cached column.  `potential_energy` forwards to the wrapped parameters.
 
 This file is the reference-side binding; it is not exercised by the C++/Python tests of
 this repository (no Julia toolchain there).
@@ -26,8 +30,9 @@ using NBodySimulator: NBodySimulation, PotentialNBodySystem, WaterSPCFw, Lennard
                       CubicPeriodicBoundaryConditions, PeriodicBoundaryConditions, InfiniteBox,
                       AndersenThermostat, BerendsenThermostat, LangevinThermostat,
                       NoseHooverThermostat, NullThermostat, SimulationResult, get_masses,
-                      MagnetostaticParameters, MagneticParticle
-import NBodySimulator: get_accelerating_function, run_simulation
+                      MagnetostaticParameters, MagneticParticle, PotentialParameters
+import NBodySimulator: get_accelerating_function, run_simulation, lennard_jones_potential,
+                       electrostatic_potential
 import SciMLBase
 using RecursiveArrayTools: ArrayPartition
 
@@ -184,7 +189,11 @@ function run_simulation(sim::NBodySimulation, alg::MI355XVerlet; dt::Real,
         check(ctx, ccall((:nbx_set_skin, libnbx), Cint, (Ptr{Cvoid}, Cdouble), ctx.ptr, alg.skin))
     u0, v0, _ = NBodySimulator.gather_bodies_initial_coordinates(sim)
     n = ctx.n
-    # a Nose-Hoover state carries an extra column (zeta); the library keeps zeta itself
+    # a Nose-Hoover state carries an extra column (u[3n+1] = zeta, v[3n+1] = its rate,
+    # src/nbody_to_ode.jl:1-20, src/thermostats.jl:120-126); the library keeps both as
+    # device scalars, and they go back into column n + 1 of every frame
+    nh = sim.thermostat isa NoseHooverThermostat
+    ncol = nh ? n + 1 : n
     u0 = Matrix{Float64}(u0[:, 1:n]); v0 = Matrix{Float64}(v0[:, 1:n])
     check(ctx, ccall((:nbx_set_state, libnbx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
                      ctx.ptr, u0, v0))
@@ -192,17 +201,48 @@ function run_simulation(sim::NBodySimulation, alg::MI355XVerlet; dt::Real,
     nsteps = round(Int, (t2 - t1) / dt)
     every = save_everystep ? 1 : max(nsteps, 1)
     nframes = 1 + nsteps ÷ every
-    uf = Array{Float64}(undef, 3, n, nframes)
-    vf = Array{Float64}(undef, 3, n, nframes)
-    check(ctx, ccall((:nbx_vv_run, libnbx), Cint,
-                     (Ptr{Cvoid}, Cdouble, Int64, UInt64, Int64, Ptr{Float64}, Ptr{Float64}),
-                     ctx.ptr, dt, nsteps, alg.seed, every, uf, vf))
+    uf = zeros(Float64, 3, ncol, nframes)
+    vf = zeros(Float64, 3, ncol, nframes)
+    if nh
+        # frame by frame (the counter-based random streams make split runs equal unsplit
+        # ones), reading zeta and xi after each piece
+        ub = Matrix{Float64}(undef, 3, n); vb = Matrix{Float64}(undef, 3, n)
+        z = Ref{Cdouble}(0.0); xi = Ref{Cdouble}(0.0)
+        for k in 1:nframes
+            k > 1 && check(ctx, ccall((:nbx_vv_run, libnbx), Cint,
+                                      (Ptr{Cvoid}, Cdouble, Int64, UInt64, Int64, Ptr{Float64},
+                                       Ptr{Float64}),
+                                      ctx.ptr, dt, every, alg.seed, 0, C_NULL, C_NULL))
+            check(ctx, ccall((:nbx_get_state, libnbx), Cint,
+                             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.ptr, ub, vb))
+            check(ctx, ccall((:nbx_get_thermostat_state, libnbx), Cint,
+                             (Ptr{Cvoid}, Ref{Cdouble}, Ref{Cdouble}), ctx.ptr, z, xi))
+            uf[:, 1:n, k] .= ub
+            vf[:, 1:n, k] .= vb
+            uf[1, n + 1, k] = z[]
+            vf[1, n + 1, k] = xi[]
+        end
+    else
+        check(ctx, ccall((:nbx_vv_run, libnbx), Cint,
+                         (Ptr{Cvoid}, Cdouble, Int64, UInt64, Int64, Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, dt, nsteps, alg.seed, every, uf, vf))
+    end
     ts = [t1 + k * every * dt for k in 0:(nframes - 1)]
     us = [ArrayPartition(vf[:, :, k], uf[:, :, k]) for k in 1:nframes]
     prob = SciMLBase.SecondOrderODEProblem(sim)
-    sol = SciMLBase.build_solution(prob, alg, ts, us)
-    sol.stats.naccept = nsteps
+    sol = SciMLBase.build_solution(prob, alg, ts, us; stats = solver_stats(nsteps),
+                                   retcode = SciMLBase.ReturnCode.Success)
     return SimulationResult(sol, sim)
+end
+
+"""Solver statistics of a GPU run: every step is accepted (fixed dt), one right-hand side
+per step plus the initial one (`SimulationResult` shows `stats.naccept`,
+src/nbody_simulation_result.jl:10-20)."""
+function solver_stats(nsteps::Integer)
+    st = SciMLBase.DEStats(0)
+    st.naccept = nsteps
+    st.nf = nsteps + 1
+    return st
 end
 
 """
@@ -236,29 +276,57 @@ function run_simulation(sim::NBodySimulation, alg::MI355XEM; dt::Real,
     ts = [t1 + k * every * dt for k in 0:(nframes - 1)]
     us = [hcat(uf[:, :, k], vf[:, :, k]) for k in 1:nframes]
     prob = SciMLBase.SDEProblem(sim)
-    sol = SciMLBase.build_solution(prob, alg, ts, us)
-    sol.stats.naccept = nsteps
+    sol = SciMLBase.build_solution(prob, alg, ts, us; stats = solver_stats(nsteps),
+                                   retcode = SciMLBase.ReturnCode.Success)
     return SimulationResult(sol, sim)
 end
 
-"""Per-atom drop-in for the closure of a potential parameter object."""
-struct MI355XPotential{P}
+"""
+    MI355XPotential(p)
+
+Per-atom drop-in for the closure of a built-in parameter object `p` (Lennard-Jones,
+electrostatic, gravitational, magnetostatic): a `PotentialParameters` that goes where `p`
+went, in `PotentialNBodySystem(bodies, Dict(:lennard_jones => MI355XPotential(p)))`.  The
+closure evaluates the FP64 reference-order kernels (bit for bit the reference's forces).
+"""
+struct MI355XPotential{P <: PotentialParameters} <: PotentialParameters
     p::P
 end
 
+# a system holding wrapped parameters also runs whole on the GPU (run_simulation above)
+add_potential!(ctx, w::MI355XPotential) = add_potential!(ctx, w.p)
+
+# energies of a system holding wrapped parameters (src/nbody_simulation_result.jl:239-351)
+lennard_jones_potential(w::MI355XPotential{<:LennardJonesParameters}, args...) =
+    lennard_jones_potential(w.p, args...)
+electrostatic_potential(w::MI355XPotential{<:ElectrostaticParameters}, args...) =
+    electrostatic_potential(w.p, args...)
+
+"""
+The closure of `w` (contract of src/nbody_to_ode.jl:110-155): `soode_system!` calls it for
+i = 1..n in order within one right-hand-side evaluation (src/nbody_to_ode.jl:474-488), so
+the call with i == 1 evaluates every atom on the GPU into a cache, and every call adds
+column i of it into `dv`.  A call that does not follow that protocol (a first call with
+i != 1, a new t, or a u whose column i differs from the cached state) refreshes the cache
+too, so direct calls stay correct.
+"""
 function get_accelerating_function(w::MI355XPotential, sim::NBodySimulation)
     ctx = build_context(sim, :fp64; only = w.p)
     n = ctx.n
     cache = zeros(3, n)
     last_u = fill(NaN, 3, n)
+    last_t = Ref{Any}(nothing)
     return (dv, u, v, t, i) -> begin
-        if u != last_u
+        stale = i == 1 || last_t[] != t || u[1, i] != last_u[1, i] ||
+                u[2, i] != last_u[2, i] || u[3, i] != last_u[3, i]
+        if stale
             fill!(cache, 0.0)
-            uu = Matrix{Float64}(u)
+            uu = Matrix{Float64}(u[:, 1:n])
             check(ctx, ccall((:nbx_accelerations, libnbx), Cint,
                              (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Int64, Ptr{Float64}),
                              ctx.ptr, 0, uu, 0, n, cache))
-            last_u .= u
+            last_u .= uu
+            last_t[] = t
         end
         @inbounds for k in 1:3
             dv[k] += cache[k, i]

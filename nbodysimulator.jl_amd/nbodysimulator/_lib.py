@@ -60,6 +60,7 @@ SIGNATURES = {
     "nbx_vv_run": (C.c_int, [_P, _D, _I64, _U64, _I64, _DP, _DP]),
     "nbx_em_run": (C.c_int, [_P, _D, _I64, _U64, _I64, _DP, _DP]),
     "nbx_steps_taken": (_I64, [_P]),
+    "nbx_get_thermostat_state": (C.c_int, [_P, _DP, _DP]),
     "nbx_energies": (C.c_int, [_P, _DP, _DP, _DP]),
     "nbx_temperature": (C.c_int, [_P, _DP]),
     "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
@@ -290,6 +291,12 @@ class Context:
 
     def steps_taken(self) -> int:
         return int(self._lib.nbx_steps_taken(self._h))
+
+    def thermostat_state(self):
+        """(zeta, xi) of a Nose-Hoover thermostat: the reference's u[3n+1], v[3n+1]."""
+        z, x = C.c_double(), C.c_double()
+        self._check(self._lib.nbx_get_thermostat_state(self._h, C.byref(z), C.byref(x)))
+        return z.value, x.value
 
     def energies(self):
         ke, pe, pg = C.c_double(), C.c_double(), C.c_double()

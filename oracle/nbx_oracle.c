@@ -489,8 +489,10 @@ static void langevin_half(const orc_system* s, double* v, double dt, int64_t ste
     }
 }
 
-int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
-                   uint64_t seed, int64_t save_every, double* u_frames, double* v_frames) {
+/* nh_out (may be NULL): the Nose-Hoover zeta and xi at the end of the run */
+int64_t orc_vv_run_nh(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
+                      uint64_t seed, int64_t save_every, double* u_frames, double* v_frames,
+                      double* nh_out) {
     const int64_t n3 = 3 * s->n;
     const size_t bytes = sizeof(double) * (size_t)(n3 > 0 ? n3 : 1);
     double* a = (double*)malloc(bytes);
@@ -524,7 +526,13 @@ int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t
         }
     }
     free(a); free(an); free(vh);
+    if (nh_out) { nh_out[0] = zeta; nh_out[1] = xi; }
     return nf;
+}
+
+int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
+                   uint64_t seed, int64_t save_every, double* u_frames, double* v_frames) {
+    return orc_vv_run_nh(s, u, v, dt, nsteps, seed, save_every, u_frames, v_frames, NULL);
 }
 
 /* Euler-Maruyama for the Langevin SDE, src/nbody_to_ode.jl:567-598.  The drift f is

@@ -97,6 +97,11 @@ void orc_rhs(const orc_system* s, const double* u, const double* v, double* dv);
  * starting with the initial one. Returns the number of frames written. */
 int64_t orc_vv_run(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
                    uint64_t seed, int64_t save_every, double* u_frames, double* v_frames);
+/* orc_vv_run that also returns the Nose-Hoover zeta and its rate xi at the end of the run
+ * (nh_out[2], may be NULL): the reference's u[3n+1], v[3n+1] */
+int64_t orc_vv_run_nh(const orc_system* s, double* u, double* v, double dt, int64_t nsteps,
+                      uint64_t seed, int64_t save_every, double* u_frames, double* v_frames,
+                      double* nh_out);
 
 /* Euler-Maruyama run of the Langevin SDE of a potential system (src/nbody_to_ode.jl:
  * 567-598 solved by StochasticDiffEq EM): with a = f(u) (the potential accelerations),

@@ -62,6 +62,8 @@ def load():
         "orc_rhs": (None, [sp, _DP, _DP, _DP]),
         "orc_vv_run": (C.c_int64, [sp, _DP, _DP, C.c_double, C.c_int64, C.c_uint64, C.c_int64,
                                    _DP, _DP]),
+        "orc_vv_run_nh": (C.c_int64, [sp, _DP, _DP, C.c_double, C.c_int64, C.c_uint64,
+                                      C.c_int64, _DP, _DP, _DP]),
         "orc_em_run": (C.c_int64, [sp, _DP, _DP, C.c_double, C.c_int64, C.c_uint64, C.c_int64,
                                    _DP, _DP]),
         "orc_kinetic_energy": (C.c_double, [_DP, _DP, C.c_int64]),
@@ -169,6 +171,15 @@ class System:
         lib.orc_vv_run(C.byref(self.s), dp(u), dp(v), dt, nsteps, seed, save_every,
                        dp(uf) if nf else None, dp(vf) if nf else None)
         return u, v, (uf[:nf], vf[:nf])
+
+    def vv_run_nh_state(self, u, v, dt, nsteps):
+        """Final (u, v) and the Nose-Hoover (zeta, xi) after nsteps steps."""
+        lib = load()
+        u = c3n(u).copy()
+        v = c3n(v).copy()
+        nh = np.zeros(2)
+        lib.orc_vv_run_nh(C.byref(self.s), dp(u), dp(v), dt, nsteps, 0, 0, None, None, dp(nh))
+        return u, v, (float(nh[0]), float(nh[1]))
 
     def em_run(self, u, v, dt, nsteps, seed=0, save_every=0):
         lib = load()

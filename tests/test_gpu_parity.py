@@ -276,6 +276,26 @@ def test_mixed_gravity_equal_and_unequal_masses():
         assert relerr(ag, ref) <= MIXED_TOL, (masses[:2], G)
 
 
+def test_mixed_gravity_si_masses_stay_finite():
+    # SI units: G = 6.67e-11, planetary to stellar masses (1e24 .. 2e30 kg) over 1e11 m.
+    # The FP32 tiles carry m / m_max, so the self term's guarded 1/r^3 times the mass stays
+    # finite (with m itself in FP32 it overflowed to inf, and 0 * inf gave NaN).
+    print("[case] mixed gravity SI masses N=2048", flush=True)
+    rng = np.random.default_rng(17)
+    n = 2048
+    masses = 10.0 ** rng.uniform(24.0, np.log10(2e30), size=n)
+    d = rng.normal(size=(n, 3))
+    u = 1e11 * d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.05, 1.0, size=(n, 1)) ** (1 / 3)
+    G = 6.67e-11
+    cg = _lib.Context(n, _lib.SYS_POTENTIAL, masses)
+    cg.add_gravitational(G)
+    cg.set_precision(_lib.PREC_MIXED)
+    ag = _mixed_accel(cg, u, np.zeros_like(u))
+    assert np.all(np.isfinite(ag))
+    ref = O.System(masses).add(O.TERM_GRAV, G).rhs(u)
+    assert relerr(ag, ref) <= MIXED_TOL
+
+
 def test_mixed_all_pairs_mode_matches(c1):
     # periodic lo/hi wrapping is not minimum image: the mixed path uses all pairs
     ctx, orc = lj_pair(864, c1.L, prec=_lib.PREC_MIXED, bc_kind=_lib.BC_PERIODIC)

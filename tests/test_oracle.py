@@ -524,3 +524,91 @@ def test_em_noise_variance_uses_first_body_mass():
     se = want * math.sqrt(2.0 / x.size)
     assert abs(x.var() - want) < 5 * se, (x.var(), want)
     assert abs(x.mean()) < 5 * math.sqrt(want / x.size)
+
+
+# ---------------------------------------------------------------------------
+# Energy conventions (src/nbody_simulation_result.jl:293-351): closed forms that fix the
+# constants, so that dropping the Lennard-Jones far-pair tail or the reaction-field
+# shift fails a test (finite-difference checks cannot see a constant)
+# ---------------------------------------------------------------------------
+
+def _U_lj(r2, eps=1.0, s2=1.0):
+    q = s2 / r2
+    return 4.0 * eps * (q ** 6 - q ** 3)
+
+
+def test_lj_energy_two_atoms_inside_and_beyond_cutoff():
+    R = 2.5
+    s = lj_system(2, R=R, eps=1.3, sigma=1.1)
+    for r in (1.05, 2.0, 2.4999):
+        u = np.array([[0.0, 0, 0], [r, 0, 0]])
+        assert s.potential_energy(u) == pytest.approx(_U_lj(r * r, 1.3, 1.21), rel=1e-14)
+    # at and beyond R every pair adds U(R), the reference's constant tail (:308-310)
+    for r in (R, 2.6, 7.0, 1e3):
+        u = np.array([[0.0, 0, 0], [0, 0, r]])
+        assert s.potential_energy(u) == pytest.approx(_U_lj(R * R, 1.3, 1.21), rel=1e-14)
+
+
+def test_lj_energy_three_atoms_near_and_far_pairs():
+    R = 2.5
+    s = lj_system(3, R=R)
+    u = np.array([[0.0, 0, 0], [1.2, 0, 0], [0, 0, 9.0]])   # one pair inside, two far
+    expect = _U_lj(1.44) + 2.0 * _U_lj(R * R)
+    assert s.potential_energy(u) == pytest.approx(expect, rel=1e-14)
+
+
+def test_lj_energy_is_continuous_across_the_cutoff():
+    R = 2.5
+    s = lj_system(2, R=R)
+    e_in = s.potential_energy(np.array([[0.0, 0, 0], [R * (1 - 1e-12), 0, 0]]))
+    e_out = s.potential_energy(np.array([[0.0, 0, 0], [R * (1 + 1e-12), 0, 0]]))
+    assert e_in == pytest.approx(e_out, rel=1e-9)
+
+
+def test_lj_energy_of_a_liquid_is_shifted_sum_plus_tail():
+    """E_ref = sum over pairs within R of (U(r) - U(R)) + n(n-1)/2 U(R), with the shifted
+    sum taken independently here (numpy, minimum image, all pairs at once)."""
+    from nbodysimulator import generators as gen
+    st = gen.lj_liquid(6, rho=0.8442, T=0.72, seed=4, jitter=0.02)   # N = 864
+    n, L, R = st.u.shape[0], st.L, 2.5
+    s = lj_system(n, bc=O.make_bc(O.BC_CUBIC, L), R=R)
+    d = st.u[:, None, :] - st.u[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    iu = np.triu_indices(n, 1)
+    r2 = r2[iu]
+    inside = r2 < R * R
+    e_shift = np.sum(_U_lj(r2[inside]) - _U_lj(R * R))
+    tail = n * (n - 1) / 2 * _U_lj(R * R)
+    # both sums cancel to ~1e-12 relative rounding; the tail is ~half the total here, so
+    # a dropped or doubled tail misses by ~50 %
+    assert s.potential_energy(st.u) == pytest.approx(e_shift + tail, rel=1e-10)
+    assert abs(tail) > 0.3 * abs(e_shift + tail)
+
+
+def _krf(R, eps_rf):
+    return 1.0 / (2 * R ** 3) if math.isinf(eps_rf) else (eps_rf - 1) / ((2 * eps_rf + 1) * R ** 3)
+
+
+@pytest.mark.parametrize("eps_rf", [78.5, math.inf])
+def test_reaction_field_pair_energy_closed_form(eps_rf):
+    k, R, q1, q2 = 138.935458, 0.9, -0.82, 0.41
+    s = O.System(np.array([15.999, 1.008]), charges=np.array([q1, q2])).add(O.TERM_RF, k, R, eps_rf)
+    krf = _krf(R, eps_rf)
+    crf = 1.0 / R + krf * R * R
+    for r in (0.25, 0.5, 0.89):
+        u = np.array([[0.0, 0, 0], [0, r, 0]])
+        expect = k * q1 * q2 * (1.0 / r + krf * r * r - crf)
+        assert s.potential_energy(u) == pytest.approx(expect, rel=1e-13)
+    # shifted to zero at the cutoff: continuous there, zero beyond
+    e_in = s.potential_energy(np.array([[0.0, 0, 0], [0, R * (1 - 1e-10), 0]]))
+    assert abs(e_in) < 1e-8 * k * abs(q1 * q2) / R
+    assert s.potential_energy(np.array([[0.0, 0, 0], [0, 1.5 * R, 0]])) == 0.0
+
+
+def test_truncated_coulomb_energy_uses_q_over_R_beyond_cutoff():
+    k, R, q1, q2 = 2.0, 1.5, 0.7, -0.3
+    s = O.System(np.array([1.0, 1.0]), charges=np.array([q1, q2])).add(O.TERM_ELEC, k, R)
+    for r, expect in ((1.0, k * q1 * q2 / 1.0), (3.0, k * q1 * q2 / R)):
+        u = np.array([[0.0, 0, 0], [r, 0, 0]])
+        assert s.potential_energy(u) == pytest.approx(expect, rel=1e-14)
