@@ -179,6 +179,18 @@ int nbx_temperature(nbx_ctx* ctx, double* T);
 int nbx_rdf(nbx_ctx* ctx, const double* frames, int64_t nframes, int64_t maxbin,
             double* rs, double* gr);
 
+/* No counterpart in the reference (it allocates freely): rdf exactly as nbx_rdf, for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  It calls no hipMalloc/hipFree: the histogram is a
+ * library-owned device array of NBX_RDF_NOALLOC_MAXBIN + 1 counters (128 KiB per GPU, held
+ * until the process ends, cleared by a fill kernel at every call) and each frame is staged in
+ * the context's own buffer.  maxbin > NBX_RDF_NOALLOC_MAXBIN fails with NBX_ERR_CAPACITY.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a GPU (from several
+ * host threads or contexts) race and must be serialised by the caller. */
+#define NBX_RDF_NOALLOC_MAXBIN 16384
+int nbx_rdf_noalloc(nbx_ctx* ctx, const double* frames, int64_t nframes, int64_t maxbin,
+                    double* rs, double* gr);
+
 /* Measurement support: average duration (ms) of the step's dominant kernel over
  * `iters` back-to-back launches on the context's stream, timed with HIP events on
  * that stream, on the current state (which is left unchanged).  *kernel_name

@@ -93,6 +93,9 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // set-up of nbx_rdf_noalloc: the device address of this GPU's histogram array
+    if (e == hipSuccess && ref64_rdf_scratch(c->device, &c->d_rdf_hist) != 0)
+        e = hipErrorInvalidSymbol;
     const size_t b3 = sizeof(double) * (size_t)(3 * (n > 0 ? n : 1));
     const size_t b1 = sizeof(double) * (size_t)(n > 0 ? n : 1);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_x, b3);
@@ -757,6 +760,23 @@ extern "C" int nbx_temperature(nbx_ctx* c, double* T) {
     return NBX_OK;
 }
 
+// g(r) and bin centres from the counts of bins 1..maxbin (src/nbody_simulation_result.jl:
+// 695-706).  counts may alias gr: count k is read before g k is written.
+static void rdf_normalize(const unsigned long long* counts, int64_t nframes, int64_t nj,
+                          double L, int64_t maxbin, double* rs, double* gr) {
+    const double dr = L / (double)maxbin;
+    const double cc = 4.0 / 3.0 * 3.141592653589793 * (double)nj / (L * L * L);
+    for (int64_t bin = 1; bin <= maxbin; ++bin) {
+        unsigned long long cnt;
+        std::memcpy(&cnt, counts + (bin - 1), sizeof(cnt));
+        double rl = (double)(bin - 1) * dr;
+        double ru = rl + dr;
+        double nideal = cc * (ru * ru * ru - rl * rl * rl);
+        gr[bin - 1] = ((double)cnt / (double)(nframes * nj)) / nideal;
+        rs[bin - 1] = rl + dr / 2.0;
+    }
+}
+
 extern "C" int nbx_rdf(nbx_ctx* c, const double* frames, int64_t nframes, int64_t maxbin,
                        double* rs, double* gr) {
     if (!c) return NBX_ERR_INVALID;
@@ -784,16 +804,43 @@ extern "C" int nbx_rdf(nbx_ctx* c, const double* frames, int64_t nframes, int64_
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     free_ptr(d_hist);
     if (e != hipSuccess) return nbx_fail(c, NBX_ERR_DEVICE, "rdf histogram copy");
-    // src/nbody_simulation_result.jl:695-706
-    const double dr = L / (double)maxbin;
-    const double cc = 4.0 / 3.0 * 3.141592653589793 * (double)nj / (L * L * L);
-    for (int64_t bin = 1; bin <= maxbin; ++bin) {
-        double rl = (double)(bin - 1) * dr;
-        double ru = rl + dr;
-        double nideal = cc * (ru * ru * ru - rl * rl * rl);
-        gr[bin - 1] = ((double)h[(size_t)bin] / (double)(nframes * nj)) / nideal;
-        rs[bin - 1] = rl + dr / 2.0;
+    rdf_normalize(h.data() + 1, nframes, nj, L, maxbin, rs, gr);
+    return NBX_OK;
+}
+
+extern "C" int nbx_rdf_noalloc(nbx_ctx* c, const double* frames, int64_t nframes,
+                               int64_t maxbin, double* rs, double* gr) {
+    if (!c) return NBX_ERR_INVALID;
+    if (c->bc.kind != NBX_BC_CUBIC)
+        return nbx_fail(c, NBX_ERR_UNSUPPORTED, "rdf needs CubicPeriodicBoundaryConditions");
+    if (!frames || nframes <= 0 || maxbin <= 0 || !rs || !gr)
+        return nbx_fail(c, NBX_ERR_INVALID, "rdf: frames, maxbin, rs and gr are required");
+    if (maxbin > NBX_RDF_NOALLOC_MAXBIN)
+        return nbx_fail(c, NBX_ERR_CAPACITY,
+                        "nbx_rdf_noalloc: maxbin " + std::to_string(maxbin) + " exceeds " +
+                            std::to_string(NBX_RDF_NOALLOC_MAXBIN) +
+                            ", the size of its fixed histogram (use nbx_rdf)");
+    if (!c->d_rdf_hist) return nbx_fail(c, NBX_ERR_DEVICE, "rdf histogram address not set up");
+    const int water = c->sys == NBX_SYS_WATER;
+    const int jstride = water ? 3 : 1;
+    const int64_t nj = water ? c->n / 3 : c->n;
+    const double L = c->bc.b[0];
+    unsigned long long* hist = c->d_rdf_hist;
+    ref64_zero_u64(c->stream, hist, maxbin + 1);
+    const size_t b = sizeof(double) * (size_t)(3 * c->n);
+    for (int64_t t = 0; t < nframes; ++t) {
+        NBX_HIP(c, hipMemcpyAsync(c->d_tmp, frames + t * 3 * c->n, b, hipMemcpyHostToDevice,
+                                  c->stream));
+        ref64_rdf_hist(c->stream, c->d_tmp, c->n, jstride, nj, L, maxbin, hist);
     }
+    NBX_HIP(c, hipGetLastError());
+    // the counts of bins 1..maxbin land in gr (8 bytes each) and are turned into g(r) there,
+    // so no host buffer is allocated either
+    static_assert(sizeof(double) == sizeof(unsigned long long), "bin counts are 8 bytes");
+    NBX_HIP(c, hipMemcpyAsync(gr, hist + 1, sizeof(unsigned long long) * (size_t)maxbin,
+                              hipMemcpyDeviceToHost, c->stream));
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
+    rdf_normalize(reinterpret_cast<const unsigned long long*>(gr), nframes, nj, L, maxbin, rs, gr);
     return NBX_OK;
 }
 

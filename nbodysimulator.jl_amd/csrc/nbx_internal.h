@@ -106,6 +106,7 @@ struct nbx_ctx {
     double* d_a = nullptr;    // accelerations at d_x, 3n
     double* d_vh = nullptr;   // half-kicked velocities, 3n
     double* d_tmp = nullptr;  // scratch 3n (host transfers, rhs)
+    unsigned long long* d_rdf_hist = nullptr;  // device address of the rdf_noalloc histogram
     double* d_ms = nullptr;
     double* d_qs = nullptr;
     double* d_mm = nullptr;   // magnetic moments (SoA 3n), when set
@@ -218,6 +219,11 @@ void ref64_bonded_pe(hipStream_t s, const double* x, int64_t n, double rOH, doub
                      double aHOH, double ka, double* d_tmp, double* d_out2);
 void ref64_rdf_hist(hipStream_t s, const double* frame_aos, int64_t n, int jstride,
                     int64_t nj, double L, int64_t maxbin, unsigned long long* d_hist);
+// address of the library-owned rdf histogram array on `device` (looked up once per device
+// with hipGetSymbolAddress, with that device current); 0 or -1
+#define NBX_MAX_DEVICES 64
+int ref64_rdf_scratch(int device, unsigned long long** out);
+void ref64_zero_u64(hipStream_t s, unsigned long long* p, int64_t count);
 int64_t ref64_check_finite(hipStream_t s, const double* p, int64_t count, int* d_flag);
 
 // ---- mixed path (nbx_mixed.cpp driving nbx_nonbonded.hip and nbx_gravity.hip) ----

@@ -5,6 +5,8 @@
 // in ascending index order (the reference's `for j in indxs`), staging sources
 // through LDS tiles; the tiling changes where operands come from, not the order
 // or the rounding of the sums.  Device state is SoA: x[k * n + i].
+#include <mutex>
+
 #include "nbx_internal.h"
 
 #define REF_TILE 256
@@ -732,6 +734,41 @@ void ref64_rdf_hist(hipStream_t s, const double* frame_aos, int64_t n, int jstri
     if (nj <= 0) return;
     hipLaunchKernelGGL(k_ref_rdf, dim3(nbx_blocks(nj, 128)), dim3(128), 0, s, frame_aos, jstride,
                        nj, L, L / (double)maxbin, maxbin, d_hist);
+}
+
+// Histogram of nbx_rdf_noalloc: library-owned device memory (bins 0..NBX_RDF_NOALLOC_MAXBIN,
+// 128 KiB), one copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel: it uses the address that hipGetSymbolAddress returned for the
+// device (ref64_rdf_scratch).
+static __device__ __align__(256) unsigned long long scratch[NBX_RDF_NOALLOC_MAXBIN + 1];
+
+int ref64_rdf_scratch(int device, unsigned long long** out) {
+    static std::mutex mu;
+    static unsigned long long* ptr[NBX_MAX_DEVICES] = {};
+    if (device < 0 || device >= NBX_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return -1;
+        if (cur != device && hipSetDevice(device) != hipSuccess) return -1;
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return -1;
+        ptr[device] = static_cast<unsigned long long*>(p);
+    }
+    *out = ptr[device];
+    return 0;
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0ull;
+}
+
+void ref64_zero_u64(hipStream_t s, unsigned long long* p, int64_t count) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nbx_blocks(count, 256)), dim3(256), 0, s, p, count);
 }
 
 __global__ void k_check_finite(const double* __restrict__ p, int64_t count, int* flag) {

@@ -21,6 +21,7 @@ NBX_ERR_UNSUPPORTED = -3
 NBX_ERR_DEVICE = -4
 NBX_ERR_CAPACITY = -5
 NBX_ERR_NONFINITE = -6
+RDF_NOALLOC_MAXBIN = 16384   # NBX_RDF_NOALLOC_MAXBIN
 
 BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
 SYS_POTENTIAL, SYS_WATER = 0, 1
@@ -64,6 +65,7 @@ SIGNATURES = {
     "nbx_energies": (C.c_int, [_P, _DP, _DP, _DP]),
     "nbx_temperature": (C.c_int, [_P, _DP]),
     "nbx_rdf": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
+    "nbx_rdf_noalloc": (C.c_int, [_P, _DP, _I64, _I64, _DP, _DP]),
     "nbx_time_force_kernel": (C.c_int, [_P, _I32, _DP, C.POINTER(C.c_char_p)]),
     "nbx_count_pairs": (C.c_int, [_P, C.POINTER(_I64)]),
     "nbx_set_kernel_timing": (C.c_int, [_P, _I32]),
@@ -315,6 +317,18 @@ class Context:
         rs = np.empty(maxbin)
         gr = np.empty(maxbin)
         self._check(self._lib.nbx_rdf(self._h, dptr(f), f.shape[0], maxbin, dptr(rs), dptr(gr)))
+        return rs, gr
+
+    def rdf_noalloc(self, frames: np.ndarray, maxbin: int = 1000):
+        """rdf through nbx_rdf_noalloc: no device allocation (library-owned histogram of
+        at most RDF_NOALLOC_MAXBIN bins)."""
+        f = np.ascontiguousarray(np.asarray(frames, dtype=np.float64))
+        if f.ndim != 3 or f.shape[1:] != (self.n, 3):
+            raise ValueError("frames must have shape (nframes, n, 3)")
+        rs = np.empty(maxbin)
+        gr = np.empty(maxbin)
+        self._check(self._lib.nbx_rdf_noalloc(self._h, dptr(f), f.shape[0], maxbin, dptr(rs),
+                                              dptr(gr)))
         return rs, gr
 
     def set_kernel_timing(self, enable: bool) -> None:
