@@ -12,7 +12,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# NBX_ORACLE_SO selects another build of the same source (the ASan/UBSan one, `make -C
+# oracle sanitize`)
+ORACLE_SO = os.environ.get("NBX_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
+if not os.path.isabs(ORACLE_SO):
+    ORACLE_SO = os.path.join(ROOT, ORACLE_SO)
 
 BC_INFINITE, BC_CUBIC, BC_PERIODIC = 0, 1, 2
 TERM_LJ, TERM_ELEC, TERM_GRAV, TERM_RF, TERM_MAG = 1, 2, 3, 4, 5
