@@ -63,6 +63,41 @@ def traffic_from_profiles(kernel: str, shape: str):
         return None
 
 
+def host_cores():
+    """Cores of the host (os.cpu_count) and those this process may run on."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"host_cores_total": os.cpu_count(), "host_cores_available": avail}
+
+
+def workload_traffic(kernel: str):
+    """HBM read + write bytes per launch of `kernel` in the committed rocprofv3 counter
+    summary of the benchmark workloads (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "workload_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def roofline(bound, kernel, work, kernel_ms, unit="TFLOP/s", peak=PEAK_FP32, **extra):
+    """Roofline object: `work` algorithmic flop (or bytes, unit GB/s) per launch divided by
+    the kernel's average launch duration, against the MI355X peak of that roof."""
+    scale = 1e12 if unit == "TFLOP/s" else 1e9
+    achieved = work / (kernel_ms * 1e-3) / scale
+    out = {"bound": bound, "kernel": kernel, "achieved": achieved, "peak": peak, "unit": unit,
+           "frac": achieved / peak, "traffic": workload_traffic(kernel),
+           "kernel_avg_ms": kernel_ms, "work_per_launch": work}
+    out.update(extra)
+    return out
+
+
 def cpu_gravity_sample(u, m, targets):
     """FP64 oracle accelerations of `targets` against all sources, and its wall time."""
     import oracle_lib as O
@@ -132,6 +167,12 @@ def bench_gravity(args, lib, gen):
     # timed run, with the tile shape (targets per thread, source chunks) from the autotuner
     ctx.set_state(u, v)
     tuned = ctx.autotune(dt, 20)
+    # the configured 100 steps from the initial state: energy drift over exactly them
+    ke_a, _, pg_a = ctx.energies()
+    ctx.vv_run(dt, 100, seed=1)
+    ke_b, _, pg_b = ctx.energies()
+    drift100 = abs((ke_b + pg_b) - (ke_a + pg_a)) / abs(ke_a + pg_a)
+    ctx.set_state(u, v)
     ke0, _, pg0 = ctx.energies()
     ctx.vv_run(dt, args.warmup, seed=1)
     # HIP event pairs around every k_grav_tiled launch of the timed steps (on the
@@ -166,8 +207,9 @@ def bench_gravity(args, lib, gen):
                                "FP64 state",
                    "n": n, "dt": dt, "steps_in_config": 100,
                    "force_error_vs_fp64_oracle": force_err,
-                   "energy_drift_rel": abs(e1 - e0) / abs(e0),
-                   "energy_drift_steps": args.warmup + args.steps,
+                   "energy_drift_rel_100_steps": drift100,
+                   "energy_drift_rel_timed_run": abs(e1 - e0) / abs(e0),
+                   "energy_drift_timed_run_steps": args.warmup + args.steps,
                    "energy_drift_rel_10k": drift10k,
                    "autotuned": {"targets_per_thread": int(tuned["shape0"]),
                                  "source_chunks": int(tuned["shape1"]),
@@ -182,7 +224,7 @@ def bench_gravity(args, lib, gen):
                      "flop_per_launch": FLOP_GRAV * pairs,
                      "issue_model": issue},
         "cpu_baseline": {"value": cpu_rate, "unit": "pair interactions/s", "cores": 1,
-                         "kind": "port",
+                         "kind": "port", **host_cores(),
                          "sample": f"FP64 oracle (reference per-atom loop restated in C, "
                                    f"-O2 -ffp-contract=off) on {len(targets)} targets x {n} "
                                    f"sources, {cpu_s:.2f} s",
@@ -223,8 +265,8 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
             rec["pairs_per_step"] = pairs
             rec["pair_interactions_per_s"] = pairs * nst / el
             kms, kname = ctx.time_force_kernel(50)
-            rec["force_kernel"] = {"name": kname, "avg_ms": kms,
-                                   "achieved_tflops": FLOP_LJ * pairs / (kms * 1e-3) / 1e12}
+            rec["roofline"] = roofline("valu", kname, FLOP_LJ * pairs, kms,
+                                       flop_per_pair=FLOP_LJ)
         out[mode[0]] = rec
     # force error on jittered positions (a perfect lattice has zero net force, so the
     # relative error there would measure rounding noise only)
@@ -239,6 +281,7 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
     cpu_step_s = cpu_s * n / len(targets)
     out["force_error_vs_fp64_oracle"] = float(np.abs(a - ref).max() / np.abs(ref).max())
     out["cpu_baseline"] = {"steps_per_s": 1.0 / cpu_step_s, "cores": 1, "kind": "port",
+                           **host_cores(),
                            "sample": f"FP64 oracle LJ accelerations of {len(targets)} of {n} "
                                      f"atoms (all-pairs, as the reference), {cpu_s:.2f} s, "
                                      f"extrapolated to one step"}
@@ -274,6 +317,19 @@ def water_context(lib, gen, w, Rc, thermostat):
     return ctx
 
 
+def water_pair_counts(u, L, rc):
+    """Ordered pairs within rc that the non-bonded kernel evaluates (every atom pair minus
+    the 6 ordered intramolecular pairs of each molecule) and ordered O-O pairs within rc
+    (the Lennard-Jones ones), from a periodic k-d tree on the host."""
+    from scipy.spatial import cKDTree
+    w = np.mod(np.asarray(u), L)
+    w[w >= L] -= L
+    n = len(w)
+    all_pairs = 2 * len(cKDTree(w, boxsize=L).query_pairs(rc))
+    oo = 2 * len(cKDTree(w[0::3], boxsize=L).query_pairs(rc))
+    return all_pairs - 6 * (n // 3), oo
+
+
 def bench_water(args, lib, gen):
     """SPC/Fw water, 4096 molecules (16^3 lattice, random orientations, 0.997 g/cm^3),
     reaction field and LJ cutoff 0.9 nm, Berendsen 300 K (tau 0.1 ps), dt 0.5 fs."""
@@ -290,6 +346,8 @@ def bench_water(args, lib, gen):
     T1 = ctx.temperature()
     pairs = ctx.count_pairs()
     kms, kname = ctx.time_force_kernel(50)
+    uw, _ = ctx.get_state()
+    p_el, p_oo = water_pair_counts(uw, w.L, Rc)
     # NVE drift of the same system over 10k steps (no thermostat)
     nve = water_context(lib, gen, w, Rc, False)
     k0, p0, _ = nve.energies()
@@ -308,10 +366,14 @@ def bench_water(args, lib, gen):
             "temperature_after_K": T1,
             "autotuned": {"lanes": int(tuned["shape0"]), "skin_nm": tuned["shape1"],
                           "ms_per_step_default": tuned["entry_ms_per_step"]},
-            "force_kernel": {"name": kname, "avg_ms": kms},
+            "roofline": roofline("valu", kname, 27 * p_el + 10 * p_oo, kms,
+                                 flop_per_pair="27 per non-excluded pair within r_c "
+                                               "(reaction field) + 10 per O-O pair (LJ)",
+                                 pairs_electrostatic=p_el, pairs_oo=p_oo),
             "nve_energy_drift_rel_10k_steps": abs((k1 + p1) - (k0 + p0)) / abs(k0 + p0),
             "force_error_vs_fp64_oracle": float(np.abs(a - ref).max() / np.abs(ref).max()),
             "cpu_baseline": {"steps_per_s": 1.0 / cpu_s, "cores": 1, "kind": "port",
+                             **host_cores(),
                              "sample": f"one FP64 oracle RHS of all {n} atoms (all pairs with "
                                        f"exclusions, as the reference), {cpu_s:.2f} s"},
             "gpu_vs_cpu_steps": (steps / el) * cpu_s}
@@ -330,7 +392,9 @@ def bench_batch(args, lib, gen):
             b.set_thermostat(k, lib.THERM_LANGEVIN, r.T, 1.0)
         else:
             b.set_thermostat(k, lib.THERM_ANDERSEN, r.T, 0.01)
-    b.set_state(np.concatenate([s.u for s in systems]), np.concatenate([s.v for s in systems]))
+    u0 = np.concatenate([s.u for s in systems])
+    v0 = np.concatenate([s.v for s in systems])
+    b.set_state(u0, v0)
     tuned = b.autotune(0.005, 200)
     steps = 5000
     b.run(0.005, 10)
@@ -349,7 +413,51 @@ def bench_batch(args, lib, gen):
     cpu_s = time.perf_counter() - t0
     n2 = np.asarray(natoms, dtype=np.float64) ** 2
     cpu_step_s = cpu_s * n2.sum() / n2[pick].sum()
-    return {"replicas": 512, "atoms": int(sum(natoms)), "steps": steps,
+    # force error: the same replicas without thermostats, one step of dt = 2^-60 from rest
+    # gives a = (v1 - v0) / dt; checked against the oracle on 8 replicas (jittered lattice)
+    offs = np.concatenate([[0], np.cumsum(natoms)])
+    uj = u0 + 0.02 * np.random.default_rng(17).standard_normal(u0.shape)
+    fb = lib.Batch(natoms, [s.L for s in systems], np.concatenate([s.masses for s in systems]))
+    fb.set_state(uj, np.zeros_like(uj))
+    dt_tiny = 2.0 ** -60
+    fb.run(dt_tiny, 1)
+    _, v1 = fb.get_state()
+    a_gpu = v1 / dt_tiny
+    errs = []
+    for k in pick:
+        s = systems[k]
+        ref = O.System(s.masses, bc=O.make_bc(O.BC_CUBIC, s.L)).add(O.TERM_LJ, 1.0, 1.0, 2.5) \
+            .rhs(uj[offs[k]:offs[k + 1]])
+        errs.append(float(np.abs(a_gpu[offs[k]:offs[k + 1]] - ref).max() / np.abs(ref).max()))
+    # NVE energy drift: the same replicas without thermostats, 10k steps
+    nve = lib.Batch(natoms, [s.L for s in systems], np.concatenate([s.masses for s in systems]))
+    nve.set_skin(tuned["skin"])
+    nve.set_state(u0, v0)
+    k0, p0, _ = nve.energies()
+    nve.run(0.005, 10000)
+    k1, p1, _ = nve.energies()
+    nat = np.asarray(natoms, dtype=np.float64)
+    tail = 0.5 * nat * (nat - 1) * 4.0 * (2.5 ** -12 - 2.5 ** -6)   # n(n-1)/2 U(R)
+    e0, e1 = k0 + p0, k1 + p1
+    drift_ref = np.abs(e1 - e0) / np.abs(e0)
+    drift_sh = np.abs(e1 - e0) / np.abs(e0 - tail)
+    ms = 1e3 * el / steps
+    atoms = int(sum(natoms))
+    return {"replicas": 512, "atoms": atoms, "steps": steps,
+            "roofline": roofline("valu", "k_batch_steps", FLOP_LJ * float(pairs.sum()), ms,
+                                 flop_per_pair=FLOP_LJ,
+                                 note="one persistent launch covers the steps: kernel time "
+                                      "per step = wall time per step",
+                                 hbm_view={"algorithmic_bytes_per_step": 120 * atoms,
+                                           "achieved_GBps": 120 * atoms / (ms * 1e-3) / 1e9,
+                                           "peak_GBps": PEAK_HBM,
+                                           "frac": 120 * atoms / (ms * 1e-3) / 1e9 / PEAK_HBM}),
+            "force_error_vs_fp64_oracle": max(errs),
+            "force_error_replicas": [int(natoms[k]) for k in pick],
+            "nve_energy_drift_rel_10k": {"median_ref_pe": float(np.median(drift_ref)),
+                                         "max_ref_pe": float(drift_ref.max()),
+                                         "median_shifted_pe": float(np.median(drift_sh)),
+                                         "max_shifted_pe": float(drift_sh.max())},
             "autotuned": {"skin": tuned["skin"], "ms_per_step_default": tuned["entry_ms_per_step"]},
             "ms_per_step": 1e3 * el / steps, "atom_steps_per_s": sum(natoms) * steps / el,
             "pairs_per_step": int(pairs.sum()),
@@ -358,6 +466,7 @@ def bench_batch(args, lib, gen):
                 th: float(np.median(np.abs(T / Tt - 1.0)[[r.thermostat == th for r in reps]]))
                 for th in ("langevin", "andersen")},
             "cpu_baseline": {"steps_per_s": 1.0 / cpu_step_s, "cores": 1, "kind": "port",
+                             **host_cores(),
                              "sample": f"FP64 oracle all-pairs RHS of 8 replicas ({cpu_s:.2f} s), "
                                        f"scaled by sum N^2 to all 512"},
             "gpu_vs_cpu_steps": (steps / el) * cpu_step_s}
@@ -426,6 +535,45 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
                 "energy_drift_rel_10k": abs(rep["E"] - rep["E0"]) / abs(rep["E0"]),
                 "fraction_of_time_strided": rep["strided_steps"] * stride / nsteps_dt}, u
 
+    def plain_stride_sweep(strides=(1, 2, 3, 4, 5, 6, 8), check_dt=100):
+        """Plain velocity Verlet at stride * dt (no network) through the production path,
+        from an equilibrated state (4,000 steps at dt from the FCC start), over the same
+        10k dt of simulated time: the largest relative deviation of the shifted total
+        energy (list energy every ~check_dt dt-steps) and simulated time per second."""
+        eq, _ = ctx_for(4242)
+        eq.set_skin(0.5)
+        eq.vv_run(dt, 4000)
+        ue, ve = eq.get_state()
+        rows = {}
+        for st in strides:
+            c, _ = ctx_for(4242)
+            c.set_skin(0.5)
+            c.set_state(ue, ve)
+            k0, p0 = c.learn_energy()
+            e0 = k0 + p0
+            every = max(1, check_dt // st)
+            total = nsteps_dt // st
+            done, maxdev, wall, err = 0, 0.0, 0.0, None
+            while done < total:
+                m = min(every, total - done)
+                t0 = time.perf_counter()
+                try:
+                    c.vv_run(dt * st, m)
+                except lib.NBXError as e:   # an exploding stride overflows its lists
+                    err = str(e)
+                    break
+                wall += time.perf_counter() - t0
+                done += m
+                k, p = c.learn_energy()
+                dev = abs(k + p - e0) / abs(e0)
+                maxdev = max(maxdev, dev if math.isfinite(dev) else math.inf)
+                if not maxdev < 0.5:
+                    break
+            rows[str(st)] = {"steps": done, "of": total, "max_rel_dev": maxdev, "error": err,
+                             "sim_time_per_s": done * st * dt / wall if wall > 0 else None}
+        # unit masses, kb = 1: T = sum v^2 / 3N
+        return {"equilibrated_T": float((ve ** 2).sum()) / (3.0 * ue.shape[0]), "strides": rows}
+
     # plain velocity Verlet at dt: the reference point and the yardstick
     plain, s = ctx_for(4242)
     plain.autotune(dt, 1000)
@@ -442,6 +590,7 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
            "plain": {"sim_time_per_s": sim_time / t_plain, "steps_per_s": nsteps_dt / t_plain,
                      "energy_drift_rel_10k": abs((ke1 + pe1) - e0) / abs(e0)},
            "strides": {}}
+    out["plain_stride_sweep"] = plain_stride_sweep()
     zero = lib.Model(0)
     zero.set(np.zeros(lib.MLP_PARAMS), np.concatenate([np.zeros(35), np.ones(35), np.ones(6)]))
     rs, g_plain = plain.rdf(up[None], 200)

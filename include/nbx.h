@@ -224,7 +224,11 @@ int nbx_valu_probe(nbx_ctx* ctx, int32_t kind, int32_t reps, double* rate);
  * plans: lanes per atom {2, 4, 8, 16, 32} x Verlet skin {0.06 .. 0.40} x cutoff; gravity:
  * targets per thread {1, 2, 4} x source chunks {1 .. 64}.  The state and step count are
  * restored.  out (nout <= 5): [0] lanes | targets per thread, [1] skin | chunks, [2] best
- * ms per step, [3] ms per step of the configuration on entry, [4] candidates timed. */
+ * ms per step, [3] ms per step of the configuration on entry, [4] candidates timed.
+ * Results are kept per (list | gravity plan, n, box side, cutoff, dt, trial_steps, system
+ * kind, thermostat, device architecture) in the process and, when the environment variable
+ * NBX_TUNE_CACHE names a file, in that file (tab-separated lines, appended); a kept result
+ * is applied without trials and reports out[4] = 0. */
 int nbx_autotune(nbx_ctx* ctx, double dt, int32_t trial_steps, double* out, int32_t nout);
 /* Diagnostics of the mixed path (no reference counterpart): out[0] Verlet-list builds
  * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),

@@ -32,8 +32,10 @@ __device__ __forceinline__ float lr_sigmoid(float z) {
     return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
 }
 
-// One 128-output layer with its epilogue, one M tile at a time (16 accumulators live):
-// acc = sum_kt W(mt, kt) Bin(kt); then per register r (unit u = 32 mt + drow(h, r)):
+// One 128-output layer with its epilogue.  The four M tiles accumulate side by side (kt
+// outer, mt inner: four independent MFMA chains in flight, 64 accumulator registers), so
+// no MFMA waits for the previous one's result; then, per M tile, acc = sum_kt W(mt, kt)
+// Bin(kt) and per register r (unit u = 32 mt + drow(h, r)):
 //   forward  (BWD = false): z = acc + bias[u], Bout = bf16(SiLU(z)); optionally z (D
 //            order) and the activation unit-major for the weight gradients
 //   backward (BWD = true) : dz = acc * SiLU'(z) with z from zsave; Bout = bf16(dz) and the
@@ -43,11 +45,17 @@ template <int KT, bool BWD>
 __device__ __forceinline__ void lr_layer(const bf16x8* __restrict__ W, int fbase, const bf16x8* Bin,
                                          bf16x8* Bout, const float* __restrict__ bias,
                                          float* zsave, uint16_t* tU, int B, int bs, int h, int lane) {
+    f32x16 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x16{};
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+            acc[mt] = lr_mfma(W[(fbase + mt * KT + kt) * 64 + lane], Bin[kt], acc[mt]);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        f32x16 a = {};
-#pragma unroll
-        for (int kt = 0; kt < KT; ++kt) a = lr_mfma(W[(fbase + mt * KT + kt) * 64 + lane], Bin[kt], a);
+        const f32x16 a = acc[mt];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             bf16x8 t;

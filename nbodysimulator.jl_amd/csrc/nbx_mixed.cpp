@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 
 #include "nbx_mixed.h"
@@ -1053,8 +1054,88 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
 //   gravity      : targets per thread {1, 2, 4} x source chunks {1, 2, 4, ..., 64}
 // out: [0] lanes | targets per thread, [1] skin | chunks, [2] best ms per step,
 //      [3] ms per step of the configuration found on entry, [4] candidates timed.
+// Results are kept per (plan kind, n, box side, cutoff, dt, trial steps, system kind,
+// thermostat, device architecture): in the process, and in the tab-separated file named by
+// NBX_TUNE_CACHE when that is set (one line per result, later lines win).  A result found
+// there is applied without trials (out[4] = 0).
 // =================================================================================
 #include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+namespace {
+
+struct tune_result {
+    double shape0, shape1, best_ms, base_ms;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, tune_result> g_tune;
+std::string g_tune_loaded;   // the cache file read into g_tune
+
+const char* tune_file() {
+    const char* f = std::getenv("NBX_TUNE_CACHE");
+    return (f && *f) ? f : nullptr;
+}
+
+std::string tune_key(nbx_ctx* c, const mx_plan& p, bool grav, double dt, int trial) {
+    hipDeviceProp_t prop{};
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    char buf[512];
+    std::snprintf(buf, sizeof(buf),
+                  "%s n=%lld L=%.17g rc=%.17g dt=%.17g trial=%d sys=%d therm=%d lj=%d el=%d arch=%s",
+                  grav ? "gravity" : "list", (long long)c->n, grav ? 0.0 : c->bc.b[0],
+                  grav ? 0.0 : p.cutoff, dt, trial, c->sys, c->therm, grav ? 0 : p.nb.has_lj,
+                  grav ? 0 : p.nb.el_kind, arch.c_str());
+    return buf;
+}
+
+// caller holds g_tune_mu
+void tune_load_file() {
+    const char* f = tune_file();
+    if (!f || g_tune_loaded == f) return;
+    g_tune_loaded = f;
+    std::ifstream in(f);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t tab = line.find('\t');
+        if (tab == std::string::npos) continue;
+        std::istringstream vals(line.substr(tab + 1));
+        tune_result r{};
+        if (vals >> r.shape0 >> r.shape1 >> r.best_ms >> r.base_ms)
+            g_tune[line.substr(0, tab)] = r;
+    }
+}
+
+bool tune_lookup(const std::string& key, tune_result* r) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    tune_load_file();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *r = it->second;
+    return true;
+}
+
+void tune_store(const std::string& key, const tune_result& r) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = r;
+    if (const char* f = tune_file()) {
+        std::ofstream o(f, std::ios::app);
+        if (o) {
+            char buf[160];
+            std::snprintf(buf, sizeof(buf), "\t%.17g\t%.17g\t%.17g\t%.17g\n", r.shape0, r.shape1,
+                          r.best_ms, r.base_ms);
+            o << key << buf;
+        }
+    }
+}
+
+}  // namespace
 
 int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     mx_plan p;
@@ -1063,6 +1144,26 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     const bool grav = p.kind == PLAN_GRAVITY;
     if (!grav && !p.nb.mode_list)
         return nbx_fail(c, NBX_ERR_UNSUPPORTED, "autotuning covers list and gravity plans");
+    const std::string key = tune_key(c, p, grav, dt, trial);
+    {
+        tune_result r{};
+        if (tune_lookup(key, &r)) {
+            if (grav) {
+                c->grav_tpt_set = (int)r.shape0;
+                c->grav_chunks_set = (int)r.shape1;
+            } else {
+                c->nl.lanes = (int)r.shape0;
+                c->skin = r.shape1;
+                c->skin_set = true;
+            }
+            c->nl.ready = false;
+            c->accel_valid = false;
+            ++c->version;
+            const double vals[5] = {r.shape0, r.shape1, r.best_ms, r.base_ms, 0.0};
+            for (int k = 0; k < nout && k < 5; ++k) out[k] = vals[k];
+            return NBX_OK;
+        }
+    }
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     hipStream_t s = c->stream;
@@ -1170,6 +1271,7 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     if ((rc = restore())) return rc;
     c->nl.ready = false;
     ++c->version;
+    tune_store(key, tune_result{best0, best1, best_ms, base_ms});
     const double vals[5] = {best0, best1, best_ms, base_ms, (double)count};
     for (int k = 0; k < nout && k < 5; ++k) out[k] = vals[k];
     return NBX_OK;

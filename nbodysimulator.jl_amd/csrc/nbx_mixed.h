@@ -147,6 +147,17 @@ __device__ __forceinline__ void mx_andersen_atom(int64_t i, double m, const mx_k
     for (int k = 0; k < 3; ++k) v3[k] = vdev * g[k];
 }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the chip, each with
+// its own L2.  This maps dispatch slot b to a logical block so that XCD x runs the x-th
+// contiguous eighth of the logical blocks: atoms that are stored near each other in space
+// then share one L2 for the positions they gather.  A bijection on [0, nb).
+#define NBX_XCDS 8
+__device__ __forceinline__ int mx_xcd_block(int b, int nb) {
+    const int q = nb / NBX_XCDS, r = nb % NBX_XCDS;
+    const int x = b % NBX_XCDS, s = b / NBX_XCDS;
+    return x * q + (x < r ? x : r) + s;
+}
+
 __device__ __forceinline__ int mx_cell_coord(double w, double inv_cs, int nc) {
     int cx = (int)(w * inv_cs);
     return cx < 0 ? 0 : (cx >= nc ? nc - 1 : cx);

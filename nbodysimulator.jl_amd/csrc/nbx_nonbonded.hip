@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(NB_THREADS)
 k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
           const int* __restrict__ nnb, int n, mx_pair_params pp, mx_kick_params kp, mx_io io,
           int* __restrict__ req_clear) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = mx_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (req_clear != nullptr && gid == 0) req_clear[0] = 0;
     mx_resolve_step(kp);
     const int i = gid / G;
@@ -625,15 +625,15 @@ __global__ void k_build_allpairs(const double4* __restrict__ pos, const double* 
     xref[2 * n + i] = x[2 * n + i];
 }
 
-void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
-                     const double* x, const int* req, int* bin_count, int* bin_atoms,
-                     int* clear_count) {
+template <bool WATER>
+static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, const double* x,
+                           const int* req, int* bin_count, int* bin_atoms, int* clear_count) {
     const nbx_nlist& nl = c->nl;
     const int n = (int)c->n;
-    if (n <= 0) return;
     const double L = c->bc.b[0];
     const int nc = nl.ncell;
     const double r2 = nl.rlist * nl.rlist;
+    int* nbr = nl.d_nbr;
     if (nc >= 3) {
         const int ncells = nc * nc * nc;
         const float r2f = (float)r2 * 1.0001f;
@@ -642,28 +642,23 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
         int parts = (512 + ncells - 1) / ncells;
         parts = parts < 1 ? 1 : (parts > 8 ? 8 : parts);
         const dim3 grid(ncells * parts);
-        if (water)
-            hipLaunchKernelGGL(k_build_cells<true>, grid, dim3(BUILD_THREADS), 0, s,
-                               (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
-                               bin_count, bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes,
-                               nl.cap, nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req,
-                               clear_count, rotate, parts);
-        else
-            hipLaunchKernelGGL(k_build_cells<false>, grid, dim3(BUILD_THREADS), 0, s,
-                               (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L),
-                               bin_count, bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes,
-                               nl.cap, nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req,
-                               clear_count, rotate, parts);
+        hipLaunchKernelGGL((k_build_cells<WATER>), grid, dim3(BUILD_THREADS), 0, s,
+                           (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L), bin_count,
+                           bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
+                           nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, rotate, parts);
     } else {
-        if (water)
-            hipLaunchKernelGGL(k_build_allpairs<true>, dim3(nbx_blocks(n, 128)), dim3(128), 0, s,
-                               (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap,
-                               nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req);
-        else
-            hipLaunchKernelGGL(k_build_allpairs<false>, dim3(nbx_blocks(n, 128)), dim3(128), 0, s,
-                               (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap,
-                               nl.d_nbr, nl.d_nnb, nl.d_xref, nl.d_flags, req);
+        hipLaunchKernelGGL((k_build_allpairs<WATER>), dim3(nbx_blocks(n, 128)), dim3(128), 0,
+                           s, (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap, nbr,
+                           nl.d_nnb, nl.d_xref, nl.d_flags, req);
     }
+}
+
+void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
+                     const double* x, const int* req, int* bin_count, int* bin_atoms,
+                     int* clear_count) {
+    if (c->n <= 0) return;
+    if (water) launch_build_t<true>(s, c, pos, x, req, bin_count, bin_atoms, clear_count);
+    else launch_build_t<false>(s, c, pos, x, req, bin_count, bin_atoms, clear_count);
 }
 
 // ---------------------------------------------------------------------------

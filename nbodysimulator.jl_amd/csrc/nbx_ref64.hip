@@ -1,10 +1,12 @@
 // nbx_ref64.hip -- FP64 kernels that reproduce the reference's arithmetic bit for bit.
 //
 // Built with -ffp-contract=off: every a*b+c rounds twice, as Julia does without
-// @fastmath/@muladd.  Each thread owns one target atom and walks the source atoms
-// in ascending index order (the reference's `for j in indxs`), staging sources
-// through LDS tiles; the tiling changes where operands come from, not the order
-// or the rounding of the sums.  Device state is SoA: x[k * n + i].
+// @fastmath/@muladd.  Every per-target sum walks the source atoms in ascending index
+// order (the reference's `for j in indxs`): gravity keeps one thread per target with
+// sources staged through LDS tiles, Lennard-Jones and Coulomb give each target a
+// wavefront whose lanes compute the pair terms of 64 sources at once before the terms
+// are added in j order.  Neither changes the order or the rounding of the sums.  Device
+// state is SoA: x[k * n + i].
 #include <mutex>
 
 #include "nbx_internal.h"
@@ -55,45 +57,72 @@ void ref64_soa_to_aos(hipStream_t s, const double* soa, double* aos, int64_t n) 
 // Lennard-Jones: src/basic_potentials.jl:240-272
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(REF_TILE)
+// Wavefront per target.  The one-thread-per-target loop leaves the chip idle at small n
+// (n = 864 is 14 wavefronts on 1,024 SIMDs), and its cost is the per-pair work: the wrap
+// loops and two FP64 divides.  Here the 64 lanes of a wavefront evaluate the pair terms
+// factor * r_ij of 64 consecutive sources j at once, and the x, y, z sums then take those
+// terms one j after another (each term read from its lane with v_readlane, the adds in
+// ascending j), so every sum sees the reference's terms in the reference's order: the
+// result is bit for bit the one-thread loop's.  A skipped pair contributes +0.0, which
+// leaves a sum unchanged (the sums start at +0.0 and never become -0.0 under
+// round-to-nearest).
+#define REF_WAVES 4   // targets (wavefronts) per block
+
+__device__ __forceinline__ double ref_lane(double v, int q) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
+    return __hiloint2double(hi, lo);
+}
+
+// f += t_q for q = 0..cnt-1 (lane q holds t_q), in q order
+__device__ __forceinline__ void ref_add_lanes(double& f1, double& f2, double& f3, double t1,
+                                              double t2, double t3, int cnt) {
+    if (cnt == 64) {
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            f1 += ref_lane(t1, q);
+            f2 += ref_lane(t2, q);
+            f3 += ref_lane(t3, q);
+        }
+    } else {
+        for (int q = 0; q < cnt; ++q) {
+            f1 += ref_lane(t1, q);
+            f2 += ref_lane(t2, q);
+            f3 += ref_lane(t3, q);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64 * REF_WAVES)
 k_ref_lj(const double* __restrict__ x, double* __restrict__ dv, const double* __restrict__ ms,
          int64_t n, int64_t ib, int64_t cnt, int istride, int jstride, int64_t nj, double s2,
          double R2, double eps, nbx_bcv bc) {
-    __shared__ double sx[REF_TILE], sy[REF_TILE], sz[REF_TILE];
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = t < cnt;
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * REF_WAVES + (threadIdx.x >> 6);
+    if (t >= cnt) return;   // the whole wavefront
     const int64_t i = ib + t * istride;
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (active) { xi = x[i]; yi = x[n + i]; zi = x[2 * n + i]; }
+    const double xi = x[i], yi = x[n + i], zi = x[2 * n + i];
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-    for (int64_t base = 0; base < nj; base += REF_TILE) {
-        const int64_t jj = base + threadIdx.x;
-        if (jj < nj) {
-            const int64_t j = jj * jstride;
-            sx[threadIdx.x] = x[j]; sy[threadIdx.x] = x[n + j]; sz[threadIdx.x] = x[2 * n + j];
-        }
-        __syncthreads();
-        const int lim = (int)((nj - base) < REF_TILE ? (nj - base) : REF_TILE);
-        if (active) {
-            for (int q = 0; q < lim; ++q) {
-                const int64_t j = (base + q) * jstride;
-                if (j == i) continue;
-                double rx, ry, rz, rij_2;
-                nbx_displacement(xi, yi, zi, sx[q], sy[q], sz[q], bc, rx, ry, rz, rij_2);
-                if (rij_2 < R2) {
-                    double qq = s2 / rij_2;
-                    double s6 = qq * qq * qq;
-                    double s12 = s6 * s6;
-                    double factor = (2.0 * s12 - s6) / rij_2;
-                    f1 += factor * rx;
-                    f2 += factor * ry;
-                    f3 += factor * rz;
-                }
+    for (int64_t base = 0; base < nj; base += 64) {
+        const int64_t jj = base + lane;
+        const int64_t j = jj * jstride;
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (jj < nj && j != i) {
+            double rx, ry, rz, rij_2;
+            nbx_displacement(xi, yi, zi, x[j], x[n + j], x[2 * n + j], bc, rx, ry, rz, rij_2);
+            if (rij_2 < R2) {
+                double qq = s2 / rij_2;
+                double s6 = qq * qq * qq;
+                double s12 = s6 * s6;
+                double factor = (2.0 * s12 - s6) / rij_2;
+                t1 = factor * rx;
+                t2 = factor * ry;
+                t3 = factor * rz;
             }
         }
-        __syncthreads();
+        ref_add_lanes(f1, f2, f3, t1, t2, t3, (int)((nj - base) < 64 ? (nj - base) : 64));
     }
-    if (active) {
+    if (lane == 0) {
         double coeff = 24.0 * eps / ms[i];
         dv[i] += coeff * f1;
         dv[n + i] += coeff * f2;
@@ -106,8 +135,8 @@ void ref64_lj(hipStream_t s, const double* x, double* dv, const double* ms, int6
               double sigma, double R, const nbx_bcv& bc) {
     if (ie <= ib) return;
     const int64_t cnt = (ie - ib + istride - 1) / istride;
-    hipLaunchKernelGGL(k_ref_lj, dim3(nbx_blocks(cnt, REF_TILE)), dim3(REF_TILE), 0, s, x, dv,
-                       ms, n, ib, cnt, istride, jstride, nj, sigma * sigma, R * R, eps, bc);
+    hipLaunchKernelGGL(k_ref_lj, dim3(nbx_blocks(cnt, REF_WAVES)), dim3(64 * REF_WAVES), 0, s, x,
+                       dv, ms, n, ib, cnt, istride, jstride, nj, sigma * sigma, R * R, eps, bc);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,45 +148,36 @@ __device__ __forceinline__ bool ref_excluded(int64_t i, int64_t j, int water) {
 }
 
 template <bool RF>
-__global__ void __launch_bounds__(REF_TILE)
+__global__ void __launch_bounds__(64 * REF_WAVES)
 k_ref_coulomb(const double* __restrict__ x, double* __restrict__ dv,
               const double* __restrict__ ms, const double* __restrict__ qs, int64_t n,
               int64_t ib, int64_t ie, int water, double k, double R2, double krf2,
               nbx_bcv bc) {
-    __shared__ double sx[REF_TILE], sy[REF_TILE], sz[REF_TILE], sq[REF_TILE];
-    const int64_t i = ib + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < ie;
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (active) { xi = x[i]; yi = x[n + i]; zi = x[2 * n + i]; }
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ib + (int64_t)blockIdx.x * REF_WAVES + (threadIdx.x >> 6);
+    if (i >= ie) return;   // the whole wavefront
+    const double xi = x[i], yi = x[n + i], zi = x[2 * n + i];
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-    for (int64_t base = 0; base < n; base += REF_TILE) {
-        const int64_t j = base + threadIdx.x;
-        if (j < n) {
-            sx[threadIdx.x] = x[j]; sy[threadIdx.x] = x[n + j];
-            sz[threadIdx.x] = x[2 * n + j]; sq[threadIdx.x] = qs[j];
-        }
-        __syncthreads();
-        const int lim = (int)((n - base) < REF_TILE ? (n - base) : REF_TILE);
-        if (active) {
-            for (int q = 0; q < lim; ++q) {
-                const int64_t jj = base + q;
-                if (ref_excluded(i, jj, water)) continue;
-                double rx, ry, rz, r2;
-                nbx_displacement(xi, yi, zi, sx[q], sy[q], sz[q], bc, rx, ry, rz, r2);
-                if (r2 < R2) {
-                    double r = sqrt(r2);
-                    double factor;
-                    if (RF) factor = sq[q] * (1.0 / (r * r2) - krf2);
-                    else factor = sq[q] / (r * r2);
-                    f1 += factor * rx;
-                    f2 += factor * ry;
-                    f3 += factor * rz;
-                }
+    for (int64_t base = 0; base < n; base += 64) {
+        const int64_t jj = base + lane;
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (jj < n && !ref_excluded(i, jj, water)) {
+            double rx, ry, rz, r2;
+            nbx_displacement(xi, yi, zi, x[jj], x[n + jj], x[2 * n + jj], bc, rx, ry, rz, r2);
+            if (r2 < R2) {
+                const double qj = qs[jj];
+                double r = sqrt(r2);
+                double factor;
+                if (RF) factor = qj * (1.0 / (r * r2) - krf2);
+                else factor = qj / (r * r2);
+                t1 = factor * rx;
+                t2 = factor * ry;
+                t3 = factor * rz;
             }
         }
-        __syncthreads();
+        ref_add_lanes(f1, f2, f3, t1, t2, t3, (int)((n - base) < 64 ? (n - base) : 64));
     }
-    if (active) {
+    if (lane == 0) {
         double coeff = k * qs[i] / ms[i];
         dv[i] += coeff * f1;
         dv[n + i] += coeff * f2;
@@ -174,17 +194,17 @@ void ref64_elec(hipStream_t s, const double* x, double* dv, const double* ms,
                 const double* qs, int64_t n, int64_t ib, int64_t ie, int water, double k,
                 double R, const nbx_bcv& bc) {
     if (ie <= ib) return;
-    hipLaunchKernelGGL(k_ref_coulomb<false>, dim3(nbx_blocks(ie - ib, REF_TILE)),
-                       dim3(REF_TILE), 0, s, x, dv, ms, qs, n, ib, ie, water, k, R * R, 0.0,
-                       bc);
+    hipLaunchKernelGGL(k_ref_coulomb<false>, dim3(nbx_blocks(ie - ib, REF_WAVES)),
+                       dim3(64 * REF_WAVES), 0, s, x, dv, ms, qs, n, ib, ie, water, k, R * R,
+                       0.0, bc);
 }
 
 void ref64_rf(hipStream_t s, const double* x, double* dv, const double* ms, const double* qs,
               int64_t n, int64_t ib, int64_t ie, int water, double k, double R,
               double eps_rf, const nbx_bcv& bc) {
     if (ie <= ib) return;
-    hipLaunchKernelGGL(k_ref_coulomb<true>, dim3(nbx_blocks(ie - ib, REF_TILE)),
-                       dim3(REF_TILE), 0, s, x, dv, ms, qs, n, ib, ie, water, k, R * R,
+    hipLaunchKernelGGL(k_ref_coulomb<true>, dim3(nbx_blocks(ie - ib, REF_WAVES)),
+                       dim3(64 * REF_WAVES), 0, s, x, dv, ms, qs, n, ib, ie, water, k, R * R,
                        2.0 * rf_krf(R, eps_rf), bc);
 }
 

@@ -48,3 +48,41 @@ def test_autotune_gravity(nbx):
     ctx.vv_run(1e-3, 100)
     ke1, _, pg1 = ctx.energies()
     assert abs((ke1 + pg1) - (ke0 + pg0)) <= 1e-5 * abs(ke0 + pg0)
+
+
+def _lj500(seed):
+    s = gen.lj_liquid(5, rho=0.8442, T=0.8, seed=seed)
+    ctx = _lib.Context(500, _lib.SYS_POTENTIAL, np.ones(500))
+    ctx.set_boundary(_lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+    ctx.add_lennard_jones(1.0, 1.0, 2.5)
+    ctx.set_kb(1.0)
+    ctx.set_precision(_lib.PREC_MIXED)
+    ctx.set_state(s.u, s.v)
+    return ctx
+
+
+def test_autotune_results_are_kept_in_process_and_file(nbx, tmp_path, monkeypatch):
+    print("[case] autotune cache LJ N=500", flush=True)
+    first = tmp_path / "tune.tsv"
+    monkeypatch.setenv("NBX_TUNE_CACHE", str(first))
+    r1 = _lj500(31).autotune(0.005, 37)
+    assert r1["candidates"] >= 20
+    lines = first.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("list n=500 ")
+    # the same configuration again (another state): the kept shape, no trials
+    c2 = _lj500(32)
+    r2 = c2.autotune(0.005, 37)
+    assert r2["candidates"] == 0
+    assert (r2["shape0"], r2["shape1"]) == (r1["shape0"], r1["shape1"])
+    assert c2.stats()["lanes"] == int(r1["shape0"])
+    # a cache file written elsewhere is read: its line overrides the kept result
+    key = lines[0].split("\t")[0]
+    other_lanes = 4 if int(r1["shape0"]) != 4 else 16
+    second = tmp_path / "other.tsv"
+    second.write_text(f"{key}\t{other_lanes}\t0.75\t0.01\t0.02\n")
+    monkeypatch.setenv("NBX_TUNE_CACHE", str(second))
+    c3 = _lj500(33)
+    r3 = c3.autotune(0.005, 37)
+    assert r3["candidates"] == 0 and r3["shape0"] == other_lanes and r3["shape1"] == 0.75
+    c3.vv_run(0.005, 50)
+    assert c3.stats()["lanes"] == other_lanes
