@@ -222,9 +222,11 @@ int nbx_valu_probe(nbx_ctx* ctx, int32_t kind, int32_t reps, double* rate);
 /* Autotuner (no reference counterpart): times `trial_steps` mixed-path steps of length dt
  * from the current state for every candidate launch shape and keeps the fastest -- list
  * plans: lanes per atom {2, 4, 8, 16, 32} x Verlet skin {0.06 .. 0.40} x cutoff; gravity:
- * targets per thread {1, 2, 4} x source chunks {1 .. 64}.  The state and step count are
- * restored.  out (nout <= 5): [0] lanes | targets per thread, [1] skin | chunks, [2] best
- * ms per step, [3] ms per step of the configuration on entry, [4] candidates timed.
+ * targets per thread {1, 2, 4} x source chunks {1 .. 64}; list plans also try list-build
+ * opportunities every 1, 2 or 5 steps.  The state and step
+ * count are restored.  out (nout <= 6): [0] lanes | targets per thread, [1] skin | chunks,
+ * [2] best ms per step, [3] ms per step of the configuration on entry, [4] candidates
+ * timed, [5] steps between list-build opportunities (1 for gravity).
  * Results are kept per (list | gravity plan, n, box side, cutoff, dt, trial_steps, system
  * kind, thermostat, device architecture) in the process and, when the environment variable
  * NBX_TUNE_CACHE names a file, in that file (tab-separated lines, appended); a kept result
@@ -232,7 +234,9 @@ int nbx_valu_probe(nbx_ctx* ctx, int32_t kind, int32_t reps, double* rate);
 int nbx_autotune(nbx_ctx* ctx, double dt, int32_t trial_steps, double* out, int32_t nout);
 /* Diagnostics of the mixed path (no reference counterpart): out[0] Verlet-list builds
  * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),
- * [3] lanes per atom in the force kernel, [4] steps per captured graph.  nout <= 5. */
+ * [3] lanes per atom in the force kernel, [4] steps per captured graph, [5] steps between
+ * list-build opportunities, [6] chunks replayed because a list went stale between two
+ * opportunities (each such replay sets [5] to 1).  nout <= 7. */
 int nbx_get_stats(nbx_ctx* ctx, int64_t* out, int32_t nout);
 
 /* ---------------------------------------------------------------------------------

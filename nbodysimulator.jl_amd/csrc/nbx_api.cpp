@@ -87,6 +87,10 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     c->bc.kind = NBX_BC_INFINITE;
     for (int k = 0; k < 6; ++k) c->bc.b[k] = (k % 2 == 0) ? -INFINITY : INFINITY;
     if (const char* g = std::getenv("NBX_GRAPH_STEPS")) c->graph_steps = std::max(0, std::atoi(g));
+    if (const char* g = std::getenv("NBX_BUILD_EVERY")) {
+        const int m = std::atoi(g);
+        if (m == 1 || m == 2 || m == 5) c->build_every = m;
+    }
     if (const char* g = std::getenv("NBX_LANES")) {
         const int l = std::atoi(g);
         if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32) c->nl.lanes = l;
@@ -936,7 +940,7 @@ extern "C" int nbx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int32_
 
 extern "C" int nbx_autotune(nbx_ctx* c, double dt, int32_t trial_steps, double* out,
                             int32_t nout) {
-    if (!c || (nout > 0 && !out) || nout < 0 || nout > 5) return NBX_ERR_INVALID;
+    if (!c || (nout > 0 && !out) || nout < 0 || nout > 6) return NBX_ERR_INVALID;
     if (!c->state_set) return nbx_fail(c, NBX_ERR_INVALID, "no state set");
     if (c->prec != NBX_PREC_MIXED)
         return nbx_fail(c, NBX_ERR_UNSUPPORTED, "autotuning is provided for the mixed path");
@@ -946,8 +950,9 @@ extern "C" int nbx_autotune(nbx_ctx* c, double dt, int32_t trial_steps, double* 
 }
 
 extern "C" int nbx_get_stats(nbx_ctx* c, int64_t* out, int32_t nout) {
-    if (!c || !out || nout < 0 || nout > 5) return NBX_ERR_INVALID;
-    int64_t v[5] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps};
+    if (!c || !out || nout < 0 || nout > 7) return NBX_ERR_INVALID;
+    int64_t v[7] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps, c->build_every,
+                    c->stale_replays};
     if (c->nl.d_flags) {
         int h[8] = {0};
         NBX_HIP(c, hipMemcpyAsync(h, c->nl.d_flags, sizeof(h), hipMemcpyDeviceToHost, c->stream));

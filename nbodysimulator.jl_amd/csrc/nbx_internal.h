@@ -45,9 +45,10 @@ struct nbx_nlist {
                                   // NBX_BINS_BUILD: three sets rotating at each build)
     int* d_cell_atoms = nullptr;  // 3 x ncells x ccap
     double* d_xref = nullptr; // unwrapped positions at the last build (SoA)
-    int* d_flags = nullptr;   // [0] / [3] rebuild request for even / odd steps,
-                              // [1] overflow, [2] builds done, [4] reduction counter,
-                              // [5] current bin set, [6] build finish counter
+    int* d_flags = nullptr;   // [0] rebuild request, [1] overflow, [2] builds done,
+                              // [4] reduction counter, [5] current bin set, [6] build
+                              // finish counter, [7] a list went stale between two build
+                              // opportunities
     size_t cap_bytes = 0;
 };
 
@@ -69,6 +70,8 @@ struct nbx_graph_key {
     const void* pos = nullptr;
     const void* part = nullptr;
     int cap = 0, ccap = 0, ncell = 0, lanes = 0, steps = 0, chunks = 0, tpt = 0;
+    int build_every = 1;      // steps between build opportunities
+    int par_off = 0;          // buffer parity of the chunk step a replay starts after
     int em = 0;               // Euler-Maruyama steps instead of velocity Verlet
     bool operator==(const nbx_graph_key& o) const {
         return valid && o.valid && version == o.version && dt == o.dt && seed == o.seed &&
@@ -76,7 +79,7 @@ struct nbx_graph_key {
                cell_atoms == o.cell_atoms && xref == o.xref && flags == o.flags && pos == o.pos &&
                part == o.part && cap == o.cap && ccap == o.ccap && ncell == o.ncell &&
                lanes == o.lanes && steps == o.steps && chunks == o.chunks && tpt == o.tpt &&
-               em == o.em;
+               em == o.em && build_every == o.build_every && par_off == o.par_off;
     }
 };
 
@@ -100,6 +103,12 @@ struct nbx_ctx {
     int em = 0;               // set while nbx_em_run steps the state (Euler-Maruyama)
     double skin = 0.0;        // user skin (when skin_set), else 0.12 x the largest cutoff
     bool skin_set = false;
+    // Verlet-list build opportunities every build_every steps (1, 2 or 5; the autotuner's
+    // choice): the steps in between launch no build or binning kernel.  An atom that moves
+    // more than skin/2 between two opportunities invalidates the chunk, which is replayed
+    // with a build opportunity at every step.
+    int build_every = 1;
+    int64_t stale_replays = 0;  // chunks replayed because a list went stale (build_every -> 1)
     // device state
     double* d_x = nullptr;    // positions (unwrapped), 3n
     double* d_v = nullptr;    // velocities, 3n

@@ -231,15 +231,14 @@ int64_t constraints(const nbx_ctx* c) { return c->sys == NBX_SYS_WATER ? 2 * (c-
 
 double* xbuf(nbx_ctx* c, int par) { return par ? c->d_x2 : c->d_x; }
 void* pbuf(nbx_ctx* c, int par) { return par ? c->d_pos2 : c->d_pos; }
-int* req_flag(nbx_ctx* c, int par) { return c->nl.d_flags + (par ? 3 : 0); }
-int* bin_count(nbx_ctx* c, int par) {
-    const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
-    return c->nl.d_cell_count ? c->nl.d_cell_count + par * ncells : nullptr;
-}
-int* bin_atoms(nbx_ctx* c, int par) {
-    const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
-    return c->nl.d_cell_atoms ? c->nl.d_cell_atoms + par * ncells * (size_t)c->nl.ccap : nullptr;
-}
+// the rebuild request: set by the drift, consumed (and cleared) by the next build
+int* req_flag(nbx_ctx* c) { return c->nl.d_flags; }
+// bins of the per-step and fixed modes (set 0; the rotating mode uses all three sets)
+int* bin_count(nbx_ctx* c) { return c->nl.d_cell_count; }
+int* bin_atoms(nbx_ctx* c) { return c->nl.d_cell_atoms; }
+
+// steps between list-build opportunities
+int build_every(const nbx_ctx* c) { return std::max(1, c->build_every); }
 
 mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
     mx_nh_params nh{};
@@ -252,8 +251,10 @@ mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
     return nh;
 }
 
-// buffers of a launch that reads x in X[par] and (when drifting) writes X[par ^ 1]
-mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0.0) {
+// buffers of a launch that reads x in X[par] and (when drifting) writes X[par ^ 1];
+// next_opp: the next step has a build opportunity
+mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0.0,
+              int next_opp = 1) {
     mx_io io{};
     io.x_in = xbuf(c, par);
     io.x_out = xbuf(c, par ^ 1);
@@ -269,12 +270,21 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
     io.flags = c->nl.d_flags;
     const bool list = p.kind != PLAN_GRAVITY && p.nb.mode_list;
     if (list) {
+        // with a build opportunity every M steps a rebuild is requested at
+        // (skin/2)(1 - (M-1)/(2M)), so that the M-1 steps to the next opportunity rarely take
+        // an atom past skin/2; one that does marks the chunk stale (replayed with M = 1)
+        const int M = build_every(c);
+        const double trig = 0.5 * c->nl.skin * (1.0 - (double)(M - 1) / (2.0 * M));
         io.xref = c->nl.d_xref;
-        io.half_skin2 = 0.25 * c->nl.skin * c->nl.skin;
+        io.half_skin2 = trig * trig;
+        io.stale2 = 0.25 * c->nl.skin * c->nl.skin;
+        io.next_opp = next_opp;
         io.guard2 = c->nl.bin_mode == NBX_BINS_BUILD ? 0.5625 * c->nl.skin * c->nl.skin
                                                       : INFINITY;
-        io.req_out = req_flag(c, par ^ 1);
+        io.req_out = req_flag(c);
         io.L = c->bc.b[0];
+    } else {
+        io.next_opp = 1;
     }
     io.need_T = ((c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER) && do_drift)
                     ? 1 : 0;
@@ -294,8 +304,7 @@ void after_drift(nbx_ctx* c, const mx_plan& p, int par_out) {
     hipStream_t s = c->stream;
     if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3 &&
         c->nl.bin_mode == NBX_BINS_STEP)
-        mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c, par_out), bin_count(c, par_out),
-                      bin_atoms(c, par_out));
+        mx_launch_bin(s, c, pbuf(c, par_out), req_flag(c), bin_count(c), bin_atoms(c));
 }
 
 // the next pair of force-kernel timing events, when timing is on and the pool has room
@@ -308,11 +317,11 @@ void timing_events(nbx_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
 }
 
 // force evaluation at x in X[par] (list: P[par]); with kp.do_kick the step's end, with
-// do_drift the start of the next step
+// do_drift the start of the next step (binned for a requested build when next_opp)
 void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par, int do_drift,
-                 bool timed = false) {
+                 bool timed = false, int next_opp = 1) {
     hipStream_t s = c->stream;
-    const mx_io io = make_io(c, p, par, do_drift, kp.dt);
+    const mx_io io = make_io(c, p, par, do_drift, kp.dt, next_opp);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) timing_events(c, &e0, &e1);
     if (p.kind == PLAN_GRAVITY) {
@@ -322,13 +331,12 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
     } else {
         mx_nb_launch nb = p.nb;
         nb.lanes = c->nl.lanes;
-        int* clear = p.nb.mode_list ? req_flag(c, par) : nullptr;
         if (e0) (void)hipEventRecord(e0, s);
-        mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io, clear);
+        mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io);
         if (e1) (void)hipEventRecord(e1, s);
         if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
     }
-    if (do_drift) after_drift(c, p, par ^ 1);
+    if (do_drift && next_opp) after_drift(c, p, par ^ 1);
 }
 
 void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
@@ -336,16 +344,10 @@ void launch_build(nbx_ctx* c, const mx_plan& p, int par) {
     // 3 cells per side: the 27 neighbour cells are the whole box, so any partition of
     // the atoms into home cells finds every neighbour -- the bins made at set-up serve
     // every build and no step bins
-    if (c->nl.bin_mode == NBX_BINS_FIXED)
-        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
-                        bin_count(c, 0), bin_atoms(c, 0), nullptr);
-    else if (c->nl.bin_mode == NBX_BINS_BUILD)
-        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
-                        c->nl.d_cell_count, c->nl.d_cell_atoms, nullptr);
-    else
-        mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c, par),
-                        bin_count(c, par), bin_atoms(c, par),
-                        c->nl.ncell >= 3 ? bin_count(c, par ^ 1) : nullptr);
+    // per-step bins are cleared by the build that used them, ready for the next binning
+    const int clear = c->nl.bin_mode == NBX_BINS_STEP && c->nl.ncell >= 3 ? 1 : 0;
+    mx_launch_build(c->stream, c, p.nb.water, pbuf(c, par), xbuf(c, par), req_flag(c),
+                    bin_count(c), bin_atoms(c), clear);
 }
 
 struct backup4 {
@@ -401,9 +403,9 @@ static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
         mx_launch_fill_int(c->stream, c->nl.d_flags + NBX_FLAG_BINSET, 2, 0);
     }
     mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, mass_wscale(c, p),
-                        list ? bin_count(c, 0) : nullptr,
-                        list ? bin_atoms(c, 0) : nullptr,
-                        (pk == 1 && bins) ? req_flag(c, 0) : nullptr);
+                        list ? bin_count(c) : nullptr,
+                        list ? bin_atoms(c) : nullptr,
+                        (pk == 1 && bins) ? req_flag(c) : nullptr);
 }
 
 // a = f(x, v) at the current state (no step taken)
@@ -474,19 +476,29 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         c->kt_ev.push_back(e);
     }
     c->kt_used = 0;
-    // steps k = 1..count of the chunk, absolute step *d_step + k; then *d_step += count
-    auto enqueue_steps = [&](int64_t k0, int64_t count) {
+    // steps k = k0+1..k0+count of the chunk, absolute step *d_step + k - k0 (the device
+    // step base is advanced by count afterwards).  Absolute steps 1, M+1, 2M+1, ... have a
+    // build opportunity (a gated list build, and the gated binning after the previous
+    // step's drift); the others launch only the force kernel.  `base` is the absolute step
+    // of k = 0 (0 when capturing a graph, whose replays start on opportunities); the
+    // drift of the chunk's last step is redone at the start of the next chunk, so it is
+    // treated as preceding an opportunity (no stale flag from a discarded state).  Step k
+    // reads the buffers of parity (k + par_off) & 1 (par_off: the parity of the chunk step
+    // a captured graph starts after).
+    auto enqueue_steps = [&](int64_t k0, int64_t count, int64_t base, int64_t last,
+                             int par_off) {
+        const int64_t M = build_every(c);
         for (int64_t k = k0 + 1; k <= k0 + count; ++k) {
-            const int par = (int)(k & 1);
+            const int par = (int)((k + par_off) & 1);
             mx_kick_params kp = kick_params(c, 1, dt, seed);
             kp.step_base = c->d_step;
             kp.step_off = k - k0;
-            launch_build(c, p, par);
-            launch_step(c, p, kp, par, 1, true);
+            if ((base + k - 1) % M == 0) launch_build(c, p, par);
+            launch_step(c, p, kp, par, 1, true, ((base + k) % M == 0 || k == last) ? 1 : 0);
         }
         mx_launch_step_advance(s, c->d_step, count, 0);
     };
-    auto graph_for = [&](int gsteps, hipGraphExec_t* out) -> int {
+    auto graph_for = [&](int gsteps, int par_off, hipGraphExec_t* out) -> int {
         nbx_graph_key key;
         key.valid = 1;
         key.version = c->version;
@@ -508,11 +520,13 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         key.ncell = c->nl.ncell;
         key.lanes = c->nl.lanes;
         key.steps = gsteps;
+        key.build_every = build_every(c);
+        key.par_off = par_off;
         if (c->gexec && key == c->gkey) { *out = c->gexec; return NBX_OK; }
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         hipGraph_t g = nullptr;
         NBX_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        enqueue_steps(0, gsteps);
+        enqueue_steps(0, gsteps, 0, gsteps, par_off);
         NBX_HIP(c, hipStreamEndCapture(s, &g));
         hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -540,35 +554,42 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         const int64_t step0 = c->steps_taken;
         for (int attempt = 0;; ++attempt) {
             mx_launch_step_advance(s, c->d_step, step0, 1);
-            // start of step 1 of the chunk: X[0] -> X[1], bins B[1], request req[1]
+            // start of step 1 of the chunk: X[0] -> X[1], binned for a requested build when
+            // step 1 has a build opportunity.  The drift after the previous chunk's last
+            // step is redone here with the same arithmetic, so its rebuild request stands
+            // (as do earlier ones still waiting for an opportunity); its bins are cleared
             if (p.kind != PLAN_GRAVITY && p.nb.mode_list && c->nl.ncell >= 3 &&
                 c->nl.bin_mode == NBX_BINS_STEP) {
                 const size_t ncells = (size_t)c->nl.ncell * c->nl.ncell * c->nl.ncell;
-                mx_launch_fill_int(s, bin_count(c, 1), (int64_t)ncells, 0);
+                mx_launch_fill_int(s, bin_count(c), (int64_t)ncells, 0);
             }
-            if (p.kind != PLAN_GRAVITY && p.nb.mode_list) mx_launch_fill_int(s, req_flag(c, 1), 1, 0);
             {
                 mx_kick_params kp = kick_params(c, 1, dt, seed);
                 kp.step_base = c->d_step;
                 kp.step_off = 0;
-                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1, dt));
-                after_drift(c, p, 1);
+                const int first_opp = (step0 % build_every(c)) == 0 ? 1 : 0;
+                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1, dt, first_opp));
+                if (first_opp) after_drift(c, p, 1);
             }
             const int gs = c->graph_steps;
-            int64_t k0 = 0;
-            if (gs > 0 && (gs % 2) == 0 && chunk >= gs && !c->kt_on) {
+            const int64_t M = build_every(c);
+            // eager steps up to the first build opportunity, so that every graph replay
+            // starts on one (a graph replays steps 1..gs of its own numbering; M divides gs)
+            int64_t k0 = std::min<int64_t>(chunk, (M - step0 % M) % M);
+            if (k0 > 0) enqueue_steps(0, k0, step0, chunk, 0);
+            if (gs > 0 && (gs % 2) == 0 && (gs % M) == 0 && chunk - k0 >= gs && !c->kt_on) {
                 hipGraphExec_t ge = nullptr;
-                if ((rc = graph_for(gs, &ge))) return rc;
+                if ((rc = graph_for(gs, (int)(k0 & 1), &ge))) return rc;
                 for (; k0 + gs <= chunk; k0 += gs) NBX_HIP(c, hipGraphLaunch(ge, s));
             }
-            if (k0 < chunk) enqueue_steps(k0, chunk - k0);
+            if (k0 < chunk) enqueue_steps(k0, chunk - k0, step0, chunk, 0);
             NBX_HIP(c, hipGetLastError());
             if (chunk & 1)  // the state is in X[1]
                 NBX_HIP(c, hipMemcpyAsync(c->d_x, c->d_x2, sizeof(double) * n3,
                                           hipMemcpyDeviceToDevice, s));
             int h[8];
             if ((rc = read_flags(c, h))) return rc;
-            if (!h[1]) {
+            if (!h[1] && !h[7]) {
                 // the chunk stands: add its force-kernel times (the stream has drained)
                 for (size_t k = 0; k + 1 < c->kt_used; k += 2) {
                     float ms = 0.f;
@@ -580,14 +601,21 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 break;
             }
             c->kt_used = 0;   // replayed: its timings are dropped
-            // overflow: restore and replay with larger lists
+            // overflow: restore and replay with larger lists; a list that went stale between
+            // two build opportunities: restore and replay with an opportunity at every step
             double* dst[4] = {c->d_x, c->d_v, c->d_a, c->d_vh};
             for (int k = 0; k < 4; ++k)
                 NBX_HIP(c, hipMemcpyAsync(dst[k], bk.p + k * n3, sizeof(double) * n3,
                                           hipMemcpyDeviceToDevice, s));
             NBX_HIP(c, hipMemcpyAsync(c->d_scal, bk.p + 4 * n3, sizeof(double) * 16,
                                       hipMemcpyDeviceToDevice, s));
-            if ((rc = grow_lists(c, p, grow))) return rc;
+            if (h[1]) {
+                if ((rc = grow_lists(c, p, grow))) return rc;
+            } else {
+                c->build_every = 1;
+                ++c->stale_replays;
+                mx_launch_fill_int(s, c->nl.d_flags + 7, 1, 0);
+            }
             if ((rc = prologue(c, p))) return rc;
             if (attempt > 8)
                 return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow replay did not converge");
@@ -631,7 +659,7 @@ int mx_time_force(nbx_ctx* c, int iters, double* avg_ms, const char** name) {
     auto launch = [&]() {
         if (p.kind == PLAN_GRAVITY) mx_launch_gravity(s, c, c->d_pos, p.G, kp, io, 1);
         else if (p.kind == PLAN_MAGNETIC) mx_launch_magnetic(s, c, c->d_pos, p.mu, kp, io, 1);
-        else mx_launch_nonbonded(s, nb, c, c->d_pos, p.pp, kp, io, nullptr);
+        else mx_launch_nonbonded(s, nb, c, c->d_pos, p.pp, kp, io);
     };
     launch();  // warm
     hipEvent_t e0, e1;
@@ -957,8 +985,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
     auto run_chunk = [&](int64_t msteps) -> int {
         int rc2;
         if ((rc2 = fresh_list(c, p))) return rc2;
-        mx_launch_fill_int(s, req_flag(c, 0), 1, 0);  // consumed by the fresh build
-        mx_launch_fill_int(s, req_flag(c, 1), 1, 0);
+        mx_launch_fill_int(s, req_flag(c), 1, 0);  // consumed by the fresh build
         start_step(0);
         const mx_kick_params kp = kick_params(c, 1, D, 0);
         for (int64_t k = 1; k <= msteps; ++k) {
@@ -1069,7 +1096,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
 namespace {
 
 struct tune_result {
-    double shape0, shape1, best_ms, base_ms;
+    double shape0, shape1, best_ms, base_ms, every;
 };
 
 std::mutex g_tune_mu;
@@ -1107,8 +1134,12 @@ void tune_load_file() {
         if (tab == std::string::npos) continue;
         std::istringstream vals(line.substr(tab + 1));
         tune_result r{};
-        if (vals >> r.shape0 >> r.shape1 >> r.best_ms >> r.base_ms)
+        r.every = 1.0;
+        if (vals >> r.shape0 >> r.shape1 >> r.best_ms >> r.base_ms) {
+            double e;
+            if (vals >> e) r.every = e;
             g_tune[line.substr(0, tab)] = r;
+        }
     }
 }
 
@@ -1127,9 +1158,9 @@ void tune_store(const std::string& key, const tune_result& r) {
     if (const char* f = tune_file()) {
         std::ofstream o(f, std::ios::app);
         if (o) {
-            char buf[160];
-            std::snprintf(buf, sizeof(buf), "\t%.17g\t%.17g\t%.17g\t%.17g\n", r.shape0, r.shape1,
-                          r.best_ms, r.base_ms);
+            char buf[200];
+            std::snprintf(buf, sizeof(buf), "\t%.17g\t%.17g\t%.17g\t%.17g\t%.17g\n", r.shape0,
+                          r.shape1, r.best_ms, r.base_ms, r.every);
             o << key << buf;
         }
     }
@@ -1155,12 +1186,13 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
                 c->nl.lanes = (int)r.shape0;
                 c->skin = r.shape1;
                 c->skin_set = true;
+                c->build_every = (int)r.every;
             }
             c->nl.ready = false;
             c->accel_valid = false;
             ++c->version;
-            const double vals[5] = {r.shape0, r.shape1, r.best_ms, r.base_ms, 0.0};
-            for (int k = 0; k < nout && k < 5; ++k) out[k] = vals[k];
+            const double vals[6] = {r.shape0, r.shape1, r.best_ms, r.base_ms, 0.0, r.every};
+            for (int k = 0; k < nout && k < 6; ++k) out[k] = vals[k];
             return NBX_OK;
         }
     }
@@ -1208,7 +1240,7 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         return NBX_OK;
     };
     double base_ms = 0.0, best_ms = 0.0;
-    double best0 = 0.0, best1 = 0.0;
+    double best0 = 0.0, best1 = 0.0, best2 = 1.0;
     int count = 0;
     if (grav) {
         const int tpt0 = c->grav_tpt_set, ch0 = c->grav_chunks_set;
@@ -1234,45 +1266,57 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         const int lanes0 = c->nl.lanes;
         const double skin0 = c->skin;
         const bool skin_set0 = c->skin_set;
+        const int every0 = c->build_every;
         if ((rc = time_config(&base_ms))) return rc;
         best_ms = base_ms;
         best0 = lanes0;
         best1 = c->nl.skin;
+        best2 = build_every(c);
         const int lanes_c[5] = {2, 4, 8, 16, 32};
         const double skin_c[7] = {0.06, 0.09, 0.12, 0.18, 0.24, 0.32, 0.40};
+        const int every_c[3] = {1, 2, 5};
         bool set_best = false;
-        for (int li = 0; li < 5; ++li)
-            for (int si = 0; si < 7; ++si) {
-                c->nl.lanes = lanes_c[li];
-                c->skin = skin_c[si] * p.cutoff;
-                c->skin_set = true;
-                double ms;
-                rc = time_config(&ms);
-                if (rc == NBX_ERR_CAPACITY) continue;  // too dense a cell for this skin
-                if (rc) return rc;
-                ++count;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best0 = lanes_c[li];
-                    best1 = c->skin;
-                    set_best = true;
+        for (int ei = 0; ei < 3; ++ei)
+            for (int li = 0; li < 5; ++li)
+                for (int si = 0; si < 7; ++si) {
+                    c->nl.lanes = lanes_c[li];
+                    c->skin = skin_c[si] * p.cutoff;
+                    c->skin_set = true;
+                    c->build_every = every_c[ei];
+                    c->stale_replays = 0;
+                    double ms;
+                    rc = time_config(&ms);
+                    if (rc == NBX_ERR_CAPACITY) continue;  // too dense a cell for this skin
+                    if (rc) return rc;
+                    ++count;
+                    // a candidate whose lists went stale ran with the fallback (every step)
+                    if (c->stale_replays) continue;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best0 = lanes_c[li];
+                        best1 = c->skin;
+                        best2 = every_c[ei];
+                        set_best = true;
+                    }
                 }
-            }
+        c->stale_replays = 0;
         if (set_best) {
             c->nl.lanes = (int)best0;
             c->skin = best1;
             c->skin_set = true;
+            c->build_every = (int)best2;
         } else {
             c->nl.lanes = lanes0;
             c->skin = skin0;
             c->skin_set = skin_set0;
+            c->build_every = every0;
         }
     }
     if ((rc = restore())) return rc;
     c->nl.ready = false;
     ++c->version;
-    tune_store(key, tune_result{best0, best1, best_ms, base_ms});
-    const double vals[5] = {best0, best1, best_ms, base_ms, (double)count};
-    for (int k = 0; k < nout && k < 5; ++k) out[k] = vals[k];
+    tune_store(key, tune_result{best0, best1, best_ms, base_ms, best2});
+    const double vals[6] = {best0, best1, best_ms, base_ms, (double)count, best2};
+    for (int k = 0; k < nout && k < 6; ++k) out[k] = vals[k];
     return NBX_OK;
 }

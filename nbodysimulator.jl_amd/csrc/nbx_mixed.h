@@ -3,14 +3,16 @@
 // nbx_mixed.cpp.
 //
 // Step k of a run, with x_k in buffer X[k & 1] and its wrapped copy in P[k & 1]:
-//   build(k)  : if req[k & 1]: Verlet list from the bins B[k & 1]; always clears B[(k+1) & 1]
+//   build(k)  : on build-opportunity steps, if req: Verlet list from the bins B; the last
+//               block to finish clears req (and, with per-step bins, the bins)
 //   step(k)   : a_k = f(x_k, vh_k); v_k = vh_k + dt a_k / 2 (+ Langevin B half step,
 //               Andersen collisions); then the start of step k+1: Langevin A half step
 //               (registers only), x_{k+1} = x_k + dt v + dt^2 a_k / 2 into X[(k+1) & 1],
 //               vh_{k+1} = v + dt a_k / 2, wrapped copy into P[(k+1) & 1], skin test
-//               against the last build -> req[(k+1) & 1], and the Berendsen temperature
-//               of vh_{k+1} (last-block reduction)
-//   bin(k+1)  : if req[(k+1) & 1]: P[(k+1) & 1] binned into B[(k+1) & 1]
+//               against the last build -> req (and the stale-list flag when step k+1 has
+//               no build opportunity), and the Berendsen temperature of vh_{k+1}
+//               (last-block reduction)
+//   bin(k+1)  : before a build opportunity, if req: P[(k+1) & 1] binned into B
 #pragma once
 #include "nbx_internal.h"
 
@@ -69,7 +71,10 @@ struct mx_io {
     int do_drift;         // 0: finish the step only (no x_{k+1})
     // Verlet-list bookkeeping (list mode)
     const double* xref;   // positions at the last build
-    double half_skin2;    // (skin/2)^2: rebuild request
+    double half_skin2;    // squared displacement that requests a rebuild: (skin/2)^2 with a
+                          // build opportunity at every step, less with fewer
+    double stale2;        // (skin/2)^2: past it the list no longer holds every pair
+    int next_opp;         // the next step has a build opportunity
     double guard2;        // NBX_BINS_BUILD: (3 skin/4)^2, past which the bins of the last
                           // build may miss pairs (flagged as an overflow)
     int* req_out;         // rebuild request for step k+1
@@ -183,6 +188,7 @@ __device__ __forceinline__ void mx_store_position(int64_t i, double px, double p
         const double e2 = ex * ex + ey * ey + ez * ez;
         if (e2 > io.half_skin2) io.req_out[0] = 1;
         if (e2 > io.guard2) io.flags[1] = 1;
+        if (!io.next_opp && e2 > io.stale2) io.flags[7] = 1;
     }
 }
 
@@ -315,17 +321,18 @@ struct mx_nb_launch {
 // the step (kp.do_kick) and drift (io.do_drift); water stores a_nb only.
 void mx_launch_nonbonded(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c,
                          const void* pos_in, const mx_pair_params& pp,
-                         const mx_kick_params& kp, const mx_io& io, int* req_clear);
+                         const mx_kick_params& kp, const mx_io& io);
 void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
                                const mx_io& io);
 // start of step kp.step + 1 from (x_in, v, a): drift of every atom
 void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
                      const mx_io& io);
-// Verlet list (gated by req) from bins B, clearing bins `clear_count` first
-// (NBX_BINS_BUILD: B is the base of the three rotating sets, which the build advances)
+// Verlet list (gated by req) from bins B; its last block clears req and, with
+// clear_bins, the counts of B (NBX_BINS_BUILD: B is the base of the three rotating sets,
+// which the build advances)
 void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
-                     const double* x, const int* req, int* bin_count, int* bin_atoms,
-                     int* clear_count);
+                     const double* x, int* req, int* bin_count, int* bin_atoms,
+                     int clear_bins);
 // wrapped copy of x into pos, binned into bins; sets *req
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
                          int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req);
@@ -342,7 +349,7 @@ void mx_launch_magnetic(hipStream_t s, const nbx_ctx* c, const void* pos, double
                         const mx_kick_params& kp, const mx_io& io, int force_only_kernel);
 void mx_launch_step_advance(hipStream_t s, int64_t* step_base, int64_t delta, int set);
 void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value);
-// bins of P (gated by *req; the bins were cleared by the preceding build kernel)
+// bins of P (gated by *req; the bins were cleared by the build kernel that used them last)
 void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* req,
                    int* bin_count, int* bin_atoms);
 // T = (sum of nparts per-block partials, in order) / denom into scal[0]; with a

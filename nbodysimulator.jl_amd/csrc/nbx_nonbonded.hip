@@ -111,10 +111,8 @@ __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* 
 template <int G, bool LJ, int EL, bool WATER>
 __global__ void __launch_bounds__(NB_THREADS)
 k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
-          const int* __restrict__ nnb, int n, mx_pair_params pp, mx_kick_params kp, mx_io io,
-          int* __restrict__ req_clear) {
+          const int* __restrict__ nnb, int n, mx_pair_params pp, mx_kick_params kp, mx_io io) {
     const int gid = mx_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (req_clear != nullptr && gid == 0) req_clear[0] = 0;
     mx_resolve_step(kp);
     const int i = gid / G;
     const int lane = gid & (G - 1);
@@ -241,26 +239,24 @@ k_nb_allpairs(int n, mx_pair_params pp, nbx_bcv bc, mx_kick_params kp, mx_io io)
 
 template <int G, bool LJ, int EL, bool WATER>
 static void launch_list(hipStream_t s, const nbx_ctx* c, const void* pos,
-                        const mx_pair_params& pp, const mx_kick_params& kp, const mx_io& io,
-                        int* req_clear) {
+                        const mx_pair_params& pp, const mx_kick_params& kp, const mx_io& io) {
     const int64_t threads = c->n * G;
     hipLaunchKernelGGL((k_nb_list<G, LJ, EL, WATER>), dim3(nbx_blocks(threads, NB_THREADS)),
                        dim3(NB_THREADS), 0, s, (const double4*)pos, c->nl.d_nbr, c->nl.d_nnb,
-                       (int)c->n, pp, kp, io, req_clear);
+                       (int)c->n, pp, kp, io);
 }
 
 template <bool LJ, int EL, bool WATER>
 static void launch_nb(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c, const void* pos,
-                      const mx_pair_params& pp, const mx_kick_params& kp, const mx_io& io,
-                      int* req_clear) {
+                      const mx_pair_params& pp, const mx_kick_params& kp, const mx_io& io) {
     if (L.mode_list) {
         switch (L.lanes) {
-        case 1: launch_list<1, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
-        case 2: launch_list<2, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
-        case 4: launch_list<4, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
-        case 16: launch_list<16, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
-        case 32: launch_list<32, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
-        default: launch_list<8, LJ, EL, WATER>(s, c, pos, pp, kp, io, req_clear); break;
+        case 1: launch_list<1, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
+        case 2: launch_list<2, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
+        case 4: launch_list<4, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
+        case 16: launch_list<16, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
+        case 32: launch_list<32, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
+        default: launch_list<8, LJ, EL, WATER>(s, c, pos, pp, kp, io); break;
         }
     } else {
         hipLaunchKernelGGL((k_nb_allpairs<LJ, EL, WATER>), dim3(nbx_blocks(c->n, AP_TILE)),
@@ -270,20 +266,20 @@ static void launch_nb(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c, co
 
 void mx_launch_nonbonded(hipStream_t s, const mx_nb_launch& L, const nbx_ctx* c,
                          const void* pos, const mx_pair_params& pp, const mx_kick_params& kp,
-                         const mx_io& io, int* req_clear) {
+                         const mx_io& io) {
     if (c->n <= 0) return;
     const int key = (L.has_lj ? 1 : 0) | (L.el_kind << 1) | (L.water ? 8 : 0);
     switch (key) {
-    case 1: launch_nb<true, 0, false>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 2: launch_nb<false, 1, false>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 3: launch_nb<true, 1, false>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 4: launch_nb<false, 2, false>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 5: launch_nb<true, 2, false>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 1 | 2 | 8: launch_nb<true, 1, true>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 1 | 4 | 8: launch_nb<true, 2, true>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 2 | 8: launch_nb<false, 1, true>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 4 | 8: launch_nb<false, 2, true>(s, L, c, pos, pp, kp, io, req_clear); break;
-    case 1 | 8: launch_nb<true, 0, true>(s, L, c, pos, pp, kp, io, req_clear); break;
+    case 1: launch_nb<true, 0, false>(s, L, c, pos, pp, kp, io); break;
+    case 2: launch_nb<false, 1, false>(s, L, c, pos, pp, kp, io); break;
+    case 3: launch_nb<true, 1, false>(s, L, c, pos, pp, kp, io); break;
+    case 4: launch_nb<false, 2, false>(s, L, c, pos, pp, kp, io); break;
+    case 5: launch_nb<true, 2, false>(s, L, c, pos, pp, kp, io); break;
+    case 1 | 2 | 8: launch_nb<true, 1, true>(s, L, c, pos, pp, kp, io); break;
+    case 1 | 4 | 8: launch_nb<true, 2, true>(s, L, c, pos, pp, kp, io); break;
+    case 2 | 8: launch_nb<false, 1, true>(s, L, c, pos, pp, kp, io); break;
+    case 4 | 8: launch_nb<false, 2, true>(s, L, c, pos, pp, kp, io); break;
+    case 1 | 8: launch_nb<true, 0, true>(s, L, c, pos, pp, kp, io); break;
     default: break;
     }
 }
@@ -444,8 +440,8 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
               float Lf, float invLf, int* __restrict__ bin_count, int* __restrict__ bin_atoms,
               int ccap, double inv_cs, float rlist2f, int G, int cap,
               int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
-              int* __restrict__ flags, const int* __restrict__ req,
-              int* __restrict__ clear_count, int rotate, int parts) {
+              int* __restrict__ flags, int* __restrict__ req, int clear_bins, int rotate,
+              int parts) {
     __shared__ int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
     __shared__ int sord[BUILD_SLOTS];
@@ -460,8 +456,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     // `parts` blocks share a home cell, each taking a contiguous slice of its atoms (more
     // blocks in flight when the cells are few and large, as for water)
     const int cid = blockIdx.x / parts, part = blockIdx.x % parts;
-    if (clear_count != nullptr && tid == 0 && part == 0) clear_count[cid] = 0;
-    if (req[0] == 0) return;
+    if (req[0] == 0) return;   // uniform: only this kernel's last block clears the request
     const int ncells = nc * nc * nc;
     // rotate: the bins of the previous build (set `bset`) are read, the bins of the
     // current positions go to the next set, and the set after it is cleared
@@ -580,27 +575,36 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
         __syncthreads();
     }
     }
-    // rotate: the last block to finish advances the set (every block read it at its start)
-    if (rotate && tid == 0) {
+    // the last block to finish (every block has read the request, the set index and its
+    // bins) clears the request and, with per-step bins, the counts of the bins it used;
+    // rotate: advances the set
+    __shared__ int last;
+    __syncthreads();
+    if (tid == 0) {
         __threadfence();
-        const unsigned done = atomicAdd(reinterpret_cast<unsigned*>(&state[1]), 1u);
-        if (done == gridDim.x - 1) {
-            state[1] = 0;
-            state[0] = (bset + 1) % 3;
-        }
+        last = atomicAdd(reinterpret_cast<unsigned*>(&state[1]), 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (clear_bins)
+        for (int c = tid; c < ncells; c += BUILD_THREADS) bin_count[c] = 0;
+    if (tid == 0) {
+        state[1] = 0;
+        if (rotate) state[0] = (bset + 1) % 3;
+        req[0] = 0;
     }
 }
 
 // small boxes (fewer than 3 cells per side): every atom scans all atoms
 template <bool WATER>
-__global__ void k_build_allpairs(const double4* __restrict__ pos, const double* __restrict__ x,
-                                 int n, double L, double invL, double rlist2, int G, int cap,
-                                 int* __restrict__ nbr, int* __restrict__ nnb,
-                                 double* __restrict__ xref, int* __restrict__ flags,
-                                 const int* __restrict__ req) {
-    if (req[0] == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ pos,
+                                                    const double* __restrict__ x, int i, int n,
+                                                    double L, double invL, double rlist2, int G,
+                                                    int cap, int* __restrict__ nbr,
+                                                    int* __restrict__ nnb,
+                                                    double* __restrict__ xref,
+                                                    int* __restrict__ flags) {
     if (i == 0) atomicAdd(&flags[2], 1);
     const double4 pi = pos[i];
     const size_t stride = (size_t)n * G;
@@ -626,8 +630,31 @@ __global__ void k_build_allpairs(const double4* __restrict__ pos, const double* 
 }
 
 template <bool WATER>
+__global__ void k_build_allpairs(const double4* __restrict__ pos, const double* __restrict__ x,
+                                 int n, double L, double invL, double rlist2, int G, int cap,
+                                 int* __restrict__ nbr, int* __restrict__ nnb,
+                                 double* __restrict__ xref, int* __restrict__ flags,
+                                 int* __restrict__ req) {
+    if (req[0] == 0) return;   // uniform: only this kernel's last block clears the request
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) build_allpairs_atom<WATER>(pos, x, i, n, L, invL, rlist2, G, cap, nbr, nnb, xref, flags);
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(reinterpret_cast<unsigned*>(&flags[NBX_FLAG_BINSET + 1]), 1u) ==
+               gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        flags[NBX_FLAG_BINSET + 1] = 0;
+        req[0] = 0;
+    }
+}
+
+template <bool WATER>
 static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, const double* x,
-                           const int* req, int* bin_count, int* bin_atoms, int* clear_count) {
+                           int* req, int* bin_count, int* bin_atoms, int clear_bins) {
     const nbx_nlist& nl = c->nl;
     const int n = (int)c->n;
     const double L = c->bc.b[0];
@@ -645,7 +672,7 @@ static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, con
         hipLaunchKernelGGL((k_build_cells<WATER>), grid, dim3(BUILD_THREADS), 0, s,
                            (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L), bin_count,
                            bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
-                           nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_count, rotate, parts);
+                           nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
     } else {
         hipLaunchKernelGGL((k_build_allpairs<WATER>), dim3(nbx_blocks(n, 128)), dim3(128), 0,
                            s, (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap, nbr,
@@ -654,11 +681,11 @@ static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, con
 }
 
 void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos,
-                     const double* x, const int* req, int* bin_count, int* bin_atoms,
-                     int* clear_count) {
+                     const double* x, int* req, int* bin_count, int* bin_atoms,
+                     int clear_bins) {
     if (c->n <= 0) return;
-    if (water) launch_build_t<true>(s, c, pos, x, req, bin_count, bin_atoms, clear_count);
-    else launch_build_t<false>(s, c, pos, x, req, bin_count, bin_atoms, clear_count);
+    if (water) launch_build_t<true>(s, c, pos, x, req, bin_count, bin_atoms, clear_bins);
+    else launch_build_t<false>(s, c, pos, x, req, bin_count, bin_atoms, clear_bins);
 }
 
 // ---------------------------------------------------------------------------

@@ -372,10 +372,10 @@ class Context:
 
     def autotune(self, dt: float, trial_steps: int = 200) -> dict:
         """Time the candidate launch shapes of the mixed path and keep the fastest."""
-        out = np.zeros(5)
-        self._check(self._lib.nbx_autotune(self._h, float(dt), int(trial_steps), dptr(out), 5))
+        out = np.zeros(6)
+        self._check(self._lib.nbx_autotune(self._h, float(dt), int(trial_steps), dptr(out), 6))
         return dict(shape0=out[0], shape1=out[1], best_ms_per_step=out[2],
-                    entry_ms_per_step=out[3], candidates=int(out[4]))
+                    entry_ms_per_step=out[3], candidates=int(out[4]), build_every=int(out[5]))
 
     # -- learned stride -------------------------------------------------------
     def features(self):
@@ -405,10 +405,11 @@ class Context:
                     list_builds=int(r[7]), trip_rel_dev=r[8])
 
     def stats(self) -> dict:
-        out = (C.c_int64 * 5)()
-        self._check(self._lib.nbx_get_stats(self._h, out, 5))
+        out = (C.c_int64 * 7)()
+        self._check(self._lib.nbx_get_stats(self._h, out, 7))
         return dict(list_builds=out[0], list_slots=out[1], cells_per_side=out[2],
-                    lanes=out[3], graph_steps=out[4])
+                    lanes=out[3], graph_steps=out[4], build_every=out[5],
+                    stale_replays=out[6])
 
 
 class Batch:
