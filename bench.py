@@ -41,6 +41,7 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 vector/matrix peak
 PEAK_HBM = 8000.0     # GB/s
+PEAK_BF16_MFMA = 2500.0  # TFLOP/s, dense BF16 MFMA
 FLOP_GRAV = 20        # per ordered interaction (GPU Gems 3 ch. 31 convention)
 FLOP_LJ = 29          # per ordered in-cutoff pair
 
@@ -72,27 +73,30 @@ def host_cores():
     return {"host_cores_total": os.cpu_count(), "host_cores_available": avail}
 
 
-def workload_traffic(kernel: str):
-    """HBM read + write bytes per launch of `kernel` in the committed rocprofv3 counter
-    summary of the benchmark workloads (None if absent)."""
+def workload_traffic(key: str, field: str = "hbm_bytes_per_launch"):
+    """A field of workload `key` (lj32000, water, batch, mlp_infer, lr_chain) in the
+    committed rocprofv3 counter summary of the benchmark workloads
+    (profiles/workload_traffic.json; None if absent)."""
     path = os.path.join(ROOT, "profiles", "workload_traffic.json")
-    if not os.path.exists(path):
+    if not key or not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+        return d.get("kernels", {}).get(key, {}).get(field)
     except (OSError, ValueError):
         return None
 
 
-def roofline(bound, kernel, work, kernel_ms, unit="TFLOP/s", peak=PEAK_FP32, **extra):
+def roofline(bound, kernel, work, kernel_ms, unit="TFLOP/s", peak=PEAK_FP32, traffic_key=None,
+             **extra):
     """Roofline object: `work` algorithmic flop (or bytes, unit GB/s) per launch divided by
-    the kernel's average launch duration, against the MI355X peak of that roof."""
+    the kernel's average launch duration, against the MI355X peak of that roof; traffic =
+    HBM bytes per launch from the committed counter summary of workload `traffic_key`."""
     scale = 1e12 if unit == "TFLOP/s" else 1e9
     achieved = work / (kernel_ms * 1e-3) / scale
     out = {"bound": bound, "kernel": kernel, "achieved": achieved, "peak": peak, "unit": unit,
-           "frac": achieved / peak, "traffic": workload_traffic(kernel),
+           "frac": achieved / peak, "traffic": workload_traffic(traffic_key),
            "kernel_avg_ms": kernel_ms, "work_per_launch": work}
     out.update(extra)
     return out
@@ -260,12 +264,14 @@ def bench_lj(args, lib, gen, cells, T, steps, cpu_targets, fp64=False):
                "energy_drift_rel": abs((ke1 + pe1) - (ke0 + pe0)) / abs(ke0 + pe0)}
         if tuned:
             rec["autotuned"] = {"lanes": int(tuned["shape0"]), "skin": tuned["shape1"],
+                                "build_every": tuned["build_every"],
                                 "ms_per_step_default": tuned["entry_ms_per_step"]}
         if pairs:
             rec["pairs_per_step"] = pairs
             rec["pair_interactions_per_s"] = pairs * nst / el
             kms, kname = ctx.time_force_kernel(50)
             rec["roofline"] = roofline("valu", kname, FLOP_LJ * pairs, kms,
+                                       traffic_key="lj32000" if n == 32000 else None,
                                        flop_per_pair=FLOP_LJ)
         out[mode[0]] = rec
     # force error on jittered positions (a perfect lattice has zero net force, so the
@@ -365,8 +371,9 @@ def bench_water(args, lib, gen):
             "pairs_per_step": pairs, "pair_interactions_per_s": pairs * steps / el,
             "temperature_after_K": T1,
             "autotuned": {"lanes": int(tuned["shape0"]), "skin_nm": tuned["shape1"],
+                          "build_every": tuned["build_every"],
                           "ms_per_step_default": tuned["entry_ms_per_step"]},
-            "roofline": roofline("valu", kname, 27 * p_el + 10 * p_oo, kms,
+            "roofline": roofline("valu", kname, 27 * p_el + 10 * p_oo, kms, traffic_key="water",
                                  flop_per_pair="27 per non-excluded pair within r_c "
                                                "(reaction field) + 10 per O-O pair (LJ)",
                                  pairs_electrostatic=p_el, pairs_oo=p_oo),
@@ -446,6 +453,7 @@ def bench_batch(args, lib, gen):
     return {"replicas": 512, "atoms": atoms, "steps": steps,
             "roofline": roofline("valu", "k_batch_steps", FLOP_LJ * float(pairs.sum()), ms,
                                  flop_per_pair=FLOP_LJ,
+                                 traffic_per_step=workload_traffic("batch", "hbm_bytes_per_step"),
                                  note="one persistent launch covers the steps: kernel time "
                                       "per step = wall time per step",
                                  hbm_view={"algorithmic_bytes_per_step": 120 * atoms,
@@ -591,6 +599,20 @@ def bench_learned(args, lib, gen, ntraj=200, traj_steps=2000, iters=4000,
                      "energy_drift_rel_10k": abs((ke1 + pe1) - e0) / abs(e0)},
            "strides": {}}
     out["plain_stride_sweep"] = plain_stride_sweep()
+    # MFMA rooflines of the learned kernels, from the committed kernel trace of the workload
+    # run (their launches are not timed inside this line)
+    rl = {}
+    for key, kname, work, what in (
+            ("mlp_infer", "k_lr_mlp_infer", 76032 * 32000,
+             "76,032 flop per atom (35-128-128-128-6 forward), 32,000 atoms"),
+            ("lr_chain", "k_lr_chain", 2 * 76032 * 8192,
+             "2 x 76,032 flop per sample (forward and backward chain), 8,192 samples")):
+        us = workload_traffic(key, "avg_us_traced")
+        if us:
+            rl[kname] = roofline("mfma", kname, work, us * 1e-3, peak=PEAK_BF16_MFMA,
+                                 traffic_key=key, work_counted=what,
+                                 timing="kernel-trace average, profiles/workload_kernel_counters.md")
+    out["rooflines"] = rl
     zero = lib.Model(0)
     zero.set(np.zeros(lib.MLP_PARAMS), np.concatenate([np.zeros(35), np.ones(35), np.ones(6)]))
     rs, g_plain = plain.rdf(up[None], 200)

@@ -717,13 +717,21 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
     const int t = threadIdx.x;
     const int i = 3 * WATER_MOLS * blockIdx.x + t;
     const bool act = i < n;
+    // every input of the atom is requested before the barrier (one round trip to memory
+    // for the whole kernel: it has 12,288 threads for water's 4,096 molecules, too few
+    // waves to hide several)
+    mx_atom A;
+    double anb[3] = {0.0, 0.0, 0.0};
+    if (act) {
+        mx_load_atom(i, n, io, A);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) anb[k] = io.a[(size_t)k * n + i];
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) sx[k][t] = act ? io.x_in[(size_t)k * n + i] : 0.0;
     __syncthreads();
     double tval = 0.0;
     if (act) {
-        mx_atom A;
-        mx_load_atom(i, n, io, A);
         const int w = t % 3;               // 0: O, 1: H1, 2: H2
         const int b = t - w;               // the molecule's O in this block
         const int o = i - w;
@@ -743,7 +751,7 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
         const double im = 1.0 / mi;
         double acc[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) acc[k] = io.a[(size_t)k * n + i] + im * f[k];
+        for (int k = 0; k < 3; ++k) acc[k] = anb[k] + im * f[k];
         // angle H1-O-H2 (src/basic_potentials.jl:395-433)
         double rba[3], rbc[3], rcb[3];
         for (int k = 0; k < 3; ++k) {
