@@ -2,11 +2,13 @@
 //
 // One workgroup of BATCH_THREADS threads owns one replica for every step of a launch:
 //   - the replica's positions, wrapped into [0, L), centred on the box and rounded to
-//     FP32, live in LDS
-//     (BATCH_NMAX float4 = 64 KB), so the pair loop never reads HBM for positions;
-//   - x, v, a, vh and the list reference positions stay in FP64 in HBM (SoA over the
-//     whole batch: component k of global atom g at k * ntot + g); a thread owns atoms
-//     tid, tid + BATCH_THREADS, ... in every phase, so it reads back its own writes;
+//     FP32, live in LDS (BATCH_NMAX float4 = 64 KB), so the pair loop never reads HBM for
+//     positions; the reference positions of the last list build live in LDS too (FP32,
+//     wrapped and centred, by atom), so the per-step displacement check reads no HBM;
+//   - x, v, a, vh stay in FP64 in HBM (SoA over the whole batch: component k of global
+//     atom g at k * ntot + g); a thread owns atoms tid, tid + BATCH_THREADS, ... in every
+//     phase, so it reads back its own writes.  The FP64 build positions are kept in HBM
+//     only to carry the reference from one launch to the next;
 //   - LDS positions are kept in cell order ("slots"); the Verlet list holds 16-bit slots
 //     in groups of 8 (entry k of local atom i at nbr[list_off + ((k / 8) * n + i) * 8 +
 //     k % 8]): a thread fetches 8 neighbours with one 16-byte load and prefetches the next
@@ -16,11 +18,16 @@
 //     are binned into an LDS cell list (cells >= list radius), placed, rank-sorted by
 //     index within each cell (reproducible lists), and every atom scans the 27
 //     neighbouring cells; replicas with fewer than 3 cells per side scan all atoms;
-//   - per step: [Langevin A] drift -> barrier -> [rebuild -> barrier] -> forces (FP32,
-//     Kahan), kick, [Langevin B | Andersen] -> barrier.  Only workgroup barriers: the
-//     replicas never synchronise with each other, so nothing crosses an XCD.
+//   - pair forces: FP32, branch-free (out-of-range pairs select a zero force factor), the
+//     8 pairs of a list group summed in FP32 and the group sums added to Kahan sums;
+//   - thermostat noise: the Philox words of the single-system path, turned into normals
+//     by Box-Muller on the FP32 hardware logarithm, square root, sine and cosine;
+//   - per step: [Langevin A] drift -> barrier -> [rebuild -> barrier] -> forces, kick,
+//     [Langevin B | Andersen] -> barrier.  Only workgroup barriers: the replicas never
+//     synchronise with each other, so nothing crosses an XCD.
 // Blocks are ordered by decreasing replica size (longest first).
 #include "nbx_internal.h"
+#include <type_traits>
 
 // 16 waves per replica (one workgroup per CU at the 128-VGPR budget): 4 waves per SIMD
 // hide the LDS and HBM latency of one replica's pair loop (8 waves measured 26 % slower
@@ -38,17 +45,19 @@ struct nbx_batch_rep {
     int32_t nc;        // cells per side of the in-kernel cell list (< 3: all pairs)
     int32_t pad;
     double inv_cs;     // nc / L
-    double L;
+    double L, invL;
     double T, param;
     double c1, c2, prob;  // per-dt thermostat coefficients (set by the host per run)
     double m0;
-    uint64_t key;         // Philox key of the replica
+    uint64_t key;
+    float c2kT;           // c2 * sqrt(kb T): the Langevin noise amplitude is c2kT / sqrt(m)         // Philox key of the replica
 };
 
 struct nbx_batch_params {
     float s2, R2, rlist2f;
     double c24eps;
     double half_skin2;
+    float half_skin2f;
     double dt, kb;
     int cap;
     int64_t ntot;
@@ -56,13 +65,40 @@ struct nbx_batch_params {
 
 // position wrapped into [0, L) in FP64, then centred on the box and rounded to FP32:
 // coordinates in [-L/2, L/2) carry half the rounding error of [0, L)
-__device__ __forceinline__ float4 batch_wrap(double x, double y, double z, double L) {
-    double wx = x - L * floor(x / L), wy = y - L * floor(y / L), wz = z - L * floor(z / L);
+__device__ __forceinline__ float4 batch_wrap(double x, double y, double z, double L,
+                                             double invL) {
+    double wx = x - L * floor(x * invL), wy = y - L * floor(y * invL),
+           wz = z - L * floor(z * invL);
     if (wx >= L) wx -= L;
     if (wy >= L) wy -= L;
     if (wz >= L) wz -= L;
+    if (wx < 0.0) wx += L;
+    if (wy < 0.0) wy += L;
+    if (wz < 0.0) wz += L;
     const double h = 0.5 * L;
     return make_float4((float)(wx - h), (float)(wy - h), (float)(wz - h), 0.0f);
+}
+
+// three standard normals from the Philox words of nbx_rng_normals3 (same counters and
+// Box-Muller pairing), the logarithm, square root, sine and cosine taken by the FP32
+// hardware instructions (v_log_f32 is log2; v_sin_f32 / v_cos_f32 take revolutions, which
+// is what the second uniform of each pair is)
+__device__ __forceinline__ void batch_normals3(uint64_t seed, int64_t atom, int64_t step,
+                                               uint32_t stream, float* g) {
+    uint32_t w[4];
+    nbx_philox((uint32_t)atom, (uint32_t)step, stream, 1u, (uint32_t)seed,
+               (uint32_t)(seed >> 32), w);
+    const float m1 = (float)(1.0 - nbx_u01(w[0], w[1]));
+    const float t1 = (float)nbx_u01(w[2], w[3]);
+    const float r1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(m1));
+    g[0] = r1 * __builtin_amdgcn_cosf(t1);
+    g[1] = r1 * __builtin_amdgcn_sinf(t1);
+    nbx_philox((uint32_t)atom, (uint32_t)step, stream, 2u, (uint32_t)seed,
+               (uint32_t)(seed >> 32), w);
+    const float m2 = (float)(1.0 - nbx_u01(w[0], w[1]));
+    const float t2 = (float)nbx_u01(w[2], w[3]);
+    const float r2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(m2));
+    g[2] = r2 * __builtin_amdgcn_cosf(t2);
 }
 
 // cell of a centred position (nc cells per side, inv_cs = nc / L, h = L / 2)
@@ -98,6 +134,8 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
     __shared__ uint16_t tidx[BATCH_NMAX];         // atom indices by slot during a build
     __shared__ int wsum[BATCH_THREADS / 64];      // wave totals of the cell-count scan
     __shared__ int rebuild;
+    // reference positions of the last build (FP32, wrapped and centred), by atom
+    __shared__ float xr0[BATCH_NMAX], xr1[BATCH_NMAX], xr2[BATCH_NMAX];
     const nbx_batch_rep rep = reps[order[blockIdx.x]];
     const int n = rep.n;
     const int tid = threadIdx.x;
@@ -162,8 +200,9 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 if (i < n) {
                     const double x0 = X0[i], x1 = X1[i], x2 = X2[i];
                     XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
-                    const int c = batch_cell(batch_wrap(x0, x1, x2, rep.L), rep.inv_cs,
-                                             0.5 * rep.L, nc);
+                    const float4 w = batch_wrap(x0, x1, x2, rep.L, rep.invL);
+                    xr0[i] = w.x; xr1[i] = w.y; xr2[i] = w.z;
+                    const int c = batch_cell(w, rep.inv_cs, 0.5 * rep.L, nc);
                     code[m] = c * BATCH_NMAX + atomicAdd(&ccount[c], 1);
                 }
             }
@@ -202,7 +241,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                     for (int u = u0; u < u1; ++u) r += tidx[u] < i;
                     slot[m] = u0 + r;
                     SL[i] = u0 + r;
-                    spos[u0 + r] = batch_wrap(X0[i], X1[i], X2[i], rep.L);
+                    spos[u0 + r] = batch_wrap(X0[i], X1[i], X2[i], rep.L, rep.invL);
                 }
             }
             __syncthreads();
@@ -271,7 +310,9 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 SL[i] = i;
                 const double x0 = X0[i], x1 = X1[i], x2 = X2[i];
                 XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
-                spos[i] = batch_wrap(x0, x1, x2, rep.L);
+                const float4 w = batch_wrap(x0, x1, x2, rep.L, rep.invL);
+                spos[i] = w;
+                xr0[i] = w.x; xr1[i] = w.y; xr2[i] = w.z;
             }
             __syncthreads();
             for (int i = tid; i < n; i += BATCH_THREADS) {
@@ -312,40 +353,53 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             const float4 pi = spos[si];
             nbx_kahan fx, fy, fz;
             fx.init(); fy.init(); fz.init();
-            auto pair = [&](const float4 pj, bool ok) {
+            // one pair, branch-free: the force factor of an out-of-range pair (or of a
+            // masked entry) is selected to zero and the products enter the group sums
+            auto pair = [&](const float4 pj, bool ok, float& gx, float& gy, float& gz) {
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 dx -= Lf * rintf(dx * invLf);
                 dy -= Lf * rintf(dy * invLf);
                 dz -= Lf * rintf(dz * invLf);
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (ok && r2 < P.R2) {
-                    const float inv = __builtin_amdgcn_rcpf(r2);
-                    const float q = P.s2 * inv;
-                    const float s6 = q * q * q;
-                    const float f = (2.0f * s6 * s6 - s6) * inv;
-                    fx.add(f * dx);
-                    fy.add(f * dy);
-                    fz.add(f * dz);
-                }
+                const float inv = __builtin_amdgcn_rcpf(r2);
+                const float q = P.s2 * inv;
+                const float s6 = q * q * q;
+                float f = (2.0f * s6 * s6 - s6) * inv;
+                f = (ok && r2 < P.R2) ? f : 0.0f;
+                gx = fmaf(f, dx, gx);
+                gy = fmaf(f, dy, gy);
+                gz = fmaf(f, dz, gz);
             };
-            // four slots of a group: positions read together, then the pairs in order
-            auto quad = [&](unsigned w0, unsigned w1, int e0, int rem) {
-                const unsigned sl[4] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu, w1 >> 16};
-                float4 pj[4];
+            // a group of 8 slots: positions read 4 at a time, the pair forces summed in FP32,
+            // the group sum added to the Kahan sums.  TAIL masks entries at or past `rem`.
+            auto group = [&](uint4 cur, int rem, auto tail) {
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
-                for (int t = 0; t < 4; ++t) pj[t] = spos[e0 + t < rem ? sl[t] : (unsigned)si];
+                for (int h = 0; h < 2; ++h) {
+                    const unsigned sl[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16,
+                                            w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
+                    float4 pj[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) pair(pj[t], e0 + t < rem);
+                    for (int t = 0; t < 4; ++t)
+                        pj[t] = spos[(decltype(tail)::value && 4 * h + t >= rem) ? (unsigned)si : sl[t]];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        pair(pj[t], decltype(tail)::value ? (4 * h + t < rem) : true, gx, gy, gz);
+                }
+                fx.add(gx);
+                fy.add(gy);
+                fz.add(gz);
             };
             const int ng = (cnt + 7) >> 3;
-            for (int g = 0; g < ng; ++g) {
+            const int nfull = cnt >> 3;
+            for (int g = 0; g < nfull; ++g) {
                 uint4 nxt = cur;
                 if (g + 1 < ng) nxt = NBG[(int64_t)(g + 1) * n + i];
-                const int rem = cnt - 8 * g;
-                quad(cur.x, cur.y, 0, rem);
-                quad(cur.z, cur.w, 4, rem);
+                group(cur, 8, std::false_type{});
                 cur = nxt;
             }
+            if (nfull < ng) group(cur, cnt - 8 * nfull, std::true_type{});
             const double c = P.c24eps / mi;
             const double ax = c * fx.value(), ay = c * fy.value(), az = c * fz.value();
             A0[i] = ax;
@@ -355,16 +409,16 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             double v3[3] = {vh3[0] + dt * (0.5 * ax), vh3[1] + dt * (0.5 * ay),
                             vh3[2] + dt * (0.5 * az)};
             if (rep.therm == NBX_THERM_LANGEVIN) {
-                const double sig = rep.c2 * sqrt(P.kb * rep.T / mi);
-                double g[3];
-                nbx_rng_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_B, g);
-                for (int k = 0; k < 3; ++k) v3[k] = rep.c1 * v3[k] + sig * g[k];
+                const float sig = rep.c2kT * __builtin_amdgcn_rsqf((float)mi);
+                float g[3];
+                batch_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_B, g);
+                for (int k = 0; k < 3; ++k) v3[k] = rep.c1 * v3[k] + (double)(sig * g[k]);
             } else if (rep.therm == NBX_THERM_ANDERSEN &&
                        nbx_rng_uniform(rep.key, i, step, NBX_RNG_ANDERSEN) < rep.prob) {
-                const double vdev = sqrt(P.kb * rep.T / rep.m0);
-                double g[3];
-                nbx_rng_normals3(rep.key, i, step, NBX_RNG_ANDERSEN, g);
-                for (int k = 0; k < 3; ++k) v3[k] = vdev * g[k];
+                const float vdev = (float)sqrt(P.kb * rep.T / rep.m0);
+                float g[3];
+                batch_normals3(rep.key, i, step, NBX_RNG_ANDERSEN, g);
+                for (int k = 0; k < 3; ++k) v3[k] = (double)(vdev * g[k]);
             }
             V0[i] = v3[0];
             V1[i] = v3[1];
@@ -372,6 +426,12 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
         }
     };
 
+    if (!prologue) {
+        for (int i = tid; i < n; i += BATCH_THREADS) {
+            const float4 w = batch_wrap(XR0[i], XR1[i], XR2[i], rep.L, rep.invL);
+            xr0[i] = w.x; xr1[i] = w.y; xr2[i] = w.z;
+        }
+    }
     if (prologue) {
         build();
         __syncthreads();
@@ -386,12 +446,11 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
         // state is loaded while the current one is computed; with it, the current atom's
         // loads are in flight during the random numbers (prefetching as well would
         // exceed the register budget of four waves per SIMD)
-        struct dstate { double v[3], a[3], x[3], r[3], m; int sl; };
+        struct dstate { double v[3], a[3], x[3], m; int sl; };
         auto dload = [&](int j, dstate& d) {
             d.v[0] = V0[j]; d.v[1] = V1[j]; d.v[2] = V2[j];
             d.a[0] = A0[j]; d.a[1] = A1[j]; d.a[2] = A2[j];
             d.x[0] = X0[j]; d.x[1] = X1[j]; d.x[2] = X2[j];
-            d.r[0] = XR0[j]; d.r[1] = XR1[j]; d.r[2] = XR2[j];
             d.m = M[j];
             d.sl = SL[j];
         };
@@ -404,20 +463,25 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             VH0[i] = vx + dt * (0.5 * ax);
             VH1[i] = vy + dt * (0.5 * ay);
             VH2[i] = vz + dt * (0.5 * az);
-            spos[d.sl] = batch_wrap(px, py, pz, rep.L);
-            const double ex = px - d.r[0], ey = py - d.r[1], ez = pz - d.r[2];
-            if (ex * ex + ey * ey + ez * ez > P.half_skin2) rebuild = 1;
+            const float4 w = batch_wrap(px, py, pz, rep.L, rep.invL);
+            spos[d.sl] = w;
+            float ex = w.x - xr0[i], ey = w.y - xr1[i], ez = w.z - xr2[i];
+            ex -= Lf * rintf(ex * invLf);
+            ey -= Lf * rintf(ey * invLf);
+            ez -= Lf * rintf(ez * invLf);
+            if (ex * ex + ey * ey + ez * ez > P.half_skin2f) rebuild = 1;
         };
         const int t0 = batch_lane();
         if (rep.therm == NBX_THERM_LANGEVIN) {
             for (int i = t0; i < n; i += BATCH_THREADS) {
                 dstate d;
                 dload(i, d);
-                double g[3];
-                nbx_rng_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_A, g);
-                const double sig = rep.c2 * sqrt(P.kb * rep.T / d.m);
-                dstore(i, d, rep.c1 * d.v[0] + sig * g[0], rep.c1 * d.v[1] + sig * g[1],
-                       rep.c1 * d.v[2] + sig * g[2]);
+                float g[3];
+                batch_normals3(rep.key, i, step, NBX_RNG_LANGEVIN_A, g);
+                const float sig = rep.c2kT * __builtin_amdgcn_rsqf((float)d.m);
+                dstore(i, d, rep.c1 * d.v[0] + (double)(sig * g[0]),
+                       rep.c1 * d.v[1] + (double)(sig * g[1]),
+                       rep.c1 * d.v[2] + (double)(sig * g[2]));
             }
         } else {
             dstate dn;
@@ -516,7 +580,8 @@ int batch_nmax() { return BATCH_NMAX; }
 
 // host-side fill of the replica records (layout private to this file)
 void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t therm, double L,
-                    double T, double param, double dt, double m0, uint64_t key, double rlist) {
+                    double T, double param, double dt, double m0, uint64_t key, double rlist,
+                    double kb) {
     nbx_batch_rep r{};
     int nc = (int)floor(L / rlist);
     while (nc * nc * nc > BATCH_MAXCELLS) --nc;
@@ -527,6 +592,7 @@ void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t
     r.n = n;
     r.therm = therm;
     r.L = L;
+    r.invL = 1.0 / L;
     r.T = T;
     r.param = param;
     r.m0 = m0;
@@ -534,6 +600,7 @@ void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t
     if (therm == NBX_THERM_LANGEVIN) {
         r.c1 = exp(-param * (0.5 * dt));
         r.c2 = sqrt(1.0 - r.c1 * r.c1);
+        r.c2kT = (float)(r.c2 * sqrt(kb * T));
     } else if (therm == NBX_THERM_ANDERSEN) {
         r.prob = param * dt;
     }
@@ -548,6 +615,7 @@ void batch_fill_params(void* dst, double eps, double sigma, double R, double rli
     P.rlist2f = (float)(rlist * rlist) * 1.0001f;
     P.c24eps = 24.0 * eps;
     P.half_skin2 = 0.25 * skin * skin;
+    P.half_skin2f = (float)P.half_skin2;
     P.dt = dt;
     P.kb = kb;
     P.cap = cap;

@@ -1,10 +1,11 @@
 """Batched replicas (one persistent launch, one workgroup per replica) against the FP64
 oracle and the single-system path.
 
-Forces: norm-wise error <= 2e-5 per replica (FP32 pair forces from LDS positions, Kahan
-sums).  Trajectories: replica r with thermostat seeds seed + r * 0x9E3779B97F4A7C15 uses
-the random streams of a single-system run with that seed, so short runs agree with the
-single-system mixed path to 1e-6."""
+Forces: norm-wise error <= 2e-5 per replica (FP32 pair forces from LDS positions, FP32
+sums of 8 pairs, Kahan sums of those).  Trajectories: replica r with thermostat seed
+seed + r * 0x9E3779B97F4A7C15 draws the Philox words of a single-system run with that seed
+(its normals come from FP32 hardware transcendentals instead of FP64 library calls), so
+short runs agree with the single-system mixed path to 1e-6."""
 import numpy as np
 import pytest
 
