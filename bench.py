@@ -448,6 +448,23 @@ def bench_batch(args, lib, gen):
     e0, e1 = k0 + p0, k1 + p1
     drift_ref = np.abs(e1 - e0) / np.abs(e0)
     drift_sh = np.abs(e1 - e0) / np.abs(e0 - tail)
+    # critical path by replica size: 256 copies of one replica (one per CU, Langevin
+    # T* = 1.0, the tuned skin); a batch's step takes as long as its slowest CU, which
+    # holds one of its largest replicas
+    per_size = {}
+    for cells in sorted({r.cells for r in reps}):
+        s1 = gen.lj_liquid(cells, rho=0.8442, T=1.0, seed=cells)
+        n1 = s1.u.shape[0]
+        ub = lib.Batch([n1] * 256, [s1.L] * 256, np.tile(s1.masses, 256))
+        ub.set_skin(tuned["skin"])
+        for k in range(256):
+            ub.set_thermostat(k, lib.THERM_LANGEVIN, 1.0, 1.0)
+        ub.set_state(np.tile(s1.u, (256, 1)), np.tile(s1.v, (256, 1)))
+        ub.run(0.005, 100)
+        t0 = time.perf_counter()
+        ub.run(0.005, 500)
+        per_size[str(n1)] = 1e3 * (time.perf_counter() - t0) / 500
+        ub.close()
     ms = 1e3 * el / steps
     atoms = int(sum(natoms))
     return {"replicas": 512, "atoms": atoms, "steps": steps,
@@ -467,6 +484,7 @@ def bench_batch(args, lib, gen):
                                          "median_shifted_pe": float(np.median(drift_sh)),
                                          "max_shifted_pe": float(drift_sh.max())},
             "autotuned": {"skin": tuned["skin"], "ms_per_step_default": tuned["entry_ms_per_step"]},
+            "ms_per_step_by_replica_size": per_size,
             "ms_per_step": 1e3 * el / steps, "atom_steps_per_s": sum(natoms) * steps / el,
             "pairs_per_step": int(pairs.sum()),
             "pair_interactions_per_s": float(pairs.sum()) * steps / el,

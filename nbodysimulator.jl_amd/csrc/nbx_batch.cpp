@@ -20,8 +20,8 @@ size_t batch_rep_size();
 size_t batch_params_size();
 int batch_nmax();
 void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t therm, double L,
-                    double T, double param, double dt, double m0, uint64_t key, double rlist,
-                    double kb);
+                    double T, double param, double dt, double m0, uint64_t key, double sigma,
+                    double R, double skin, double kb);
 void batch_fill_params(void* dst, double eps, double sigma, double R, double rlist, double skin,
                        double dt, double kb, int cap, int64_t ntot);
 void batch_launch_steps_raw(hipStream_t s, int nblocks, const void* reps, const int* order,
@@ -294,7 +294,7 @@ static int upload_tables(nbx_batch* b, double dt, uint64_t seed) {
         const uint64_t key = seed + (uint64_t)r * 0x9E3779B97F4A7C15ull;
         batch_fill_rep(reps.data() + batch_rep_size() * r, b->off[r], b->list_off[r],
                        (int32_t)b->n[r], b->therm[r], b->L[r], b->T[r], b->param[r], dt,
-                       b->m0[r], key, rlist_of(b), b->kb);
+                       b->m0[r], key, b->sigma, b->R, b->skin, b->kb);
     }
     BHIP(b, hipMemcpyAsync(b->d_reps, reps.data(), reps.size(), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -440,7 +440,7 @@ extern "C" int nbx_batch_energies(nbx_batch* b, double* ke, double* pe, int64_t*
     for (int r = 0; r < b->nrep; ++r)
         batch_fill_rep(reps.data() + batch_rep_size() * r, b->off[r], b->list_off[r],
                        (int32_t)b->n[r], b->therm[r], b->L[r], b->T[r], b->param[r], 1.0,
-                       b->m0[r], 0, rlist_of(b), b->kb);
+                       b->m0[r], 0, b->sigma, b->R, b->skin, b->kb);
     void* d_reps = nullptr;
     double* d_out = nullptr;
     BHIP(b, hipMalloc(&d_reps, reps.size()));

@@ -1,10 +1,13 @@
 // nbx_batch.hip -- batched Lennard-Jones replicas in one persistent launch (gfx950).
 //
 // One workgroup of BATCH_THREADS threads owns one replica for every step of a launch:
-//   - the replica's positions, wrapped into [0, L), centred on the box and rounded to
-//     FP32, live in LDS (BATCH_NMAX float4 = 64 KB), so the pair loop never reads HBM for
-//     positions; the reference positions of the last list build live in LDS too (FP32,
-//     wrapped and centred, by atom), so the per-step displacement check reads no HBM;
+//   - the replica's positions, wrapped into [0, L), centred on the box, in units of the
+//     box side (fractional, [-1/2, 1/2)) and rounded to FP32, live in LDS (BATCH_NMAX
+//     float4 = 64 KB), so the pair loop never reads HBM for positions; the minimum image
+//     of a fractional difference d is d - rint(d), and the cutoffs and the LJ
+//     coefficients are scaled per replica to box units; the reference positions of the
+//     last list build live in LDS too (fractional FP32, by atom), so the per-step
+//     displacement check reads no HBM;
 //   - x, v, a, vh stay in FP64 in HBM (SoA over the whole batch: component k of global
 //     atom g at k * ntot + g); a thread owns atoms tid, tid + BATCH_THREADS, ... in every
 //     phase, so it reads back its own writes.  The FP64 build positions are kept in HBM
@@ -44,8 +47,10 @@ struct nbx_batch_rep {
     int32_t therm;
     int32_t nc;        // cells per side of the in-kernel cell list (< 3: all pairs)
     int32_t pad;
-    double inv_cs;     // nc / L
     double L, invL;
+    // in units of the box side (positions in LDS are fractional): (sigma/L)^6,
+    // 2 (sigma/L)^12, (R/L)^2, list radius^2 (with a 1e-4 margin), (skin/2)^2
+    float c6, c12, R2, rlist2, hs2;
     double T, param;
     double c1, c2, prob;  // per-dt thermostat coefficients (set by the host per run)
     double m0;
@@ -54,10 +59,7 @@ struct nbx_batch_rep {
 };
 
 struct nbx_batch_params {
-    float s2, R2, rlist2f;
     double c24eps;
-    double half_skin2;
-    float half_skin2f;
     double dt, kb;
     int cap;
     int64_t ntot;
@@ -75,8 +77,8 @@ __device__ __forceinline__ float4 batch_wrap(double x, double y, double z, doubl
     if (wx < 0.0) wx += L;
     if (wy < 0.0) wy += L;
     if (wz < 0.0) wz += L;
-    const double h = 0.5 * L;
-    return make_float4((float)(wx - h), (float)(wy - h), (float)(wz - h), 0.0f);
+    return make_float4((float)(wx * invL - 0.5), (float)(wy * invL - 0.5),
+                       (float)(wz * invL - 0.5), 0.0f);
 }
 
 // three standard normals from the Philox words of nbx_rng_normals3 (same counters and
@@ -101,10 +103,11 @@ __device__ __forceinline__ void batch_normals3(uint64_t seed, int64_t atom, int6
     g[2] = r2 * __builtin_amdgcn_cosf(t2);
 }
 
-// cell of a centred position (nc cells per side, inv_cs = nc / L, h = L / 2)
-__device__ __forceinline__ int batch_cell(float4 w, double inv_cs, double h, int nc) {
-    int cx = (int)((w.x + h) * inv_cs), cy = (int)((w.y + h) * inv_cs),
-        cz = (int)((w.z + h) * inv_cs);
+// cell of a centred fractional position (nc cells per side)
+__device__ __forceinline__ int batch_cell(float4 w, int nc) {
+    const float fn = (float)nc;
+    int cx = (int)((w.x + 0.5f) * fn), cy = (int)((w.y + 0.5f) * fn),
+        cz = (int)((w.z + 0.5f) * fn);
     cx = cx < 0 ? 0 : (cx < nc ? cx : nc - 1);
     cy = cy < 0 ? 0 : (cy < nc ? cy : nc - 1);
     cz = cz < 0 ? 0 : (cz < nc ? cz : nc - 1);
@@ -152,7 +155,6 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
     const uint4* NBG = reinterpret_cast<const uint4*>(NB);  // group g of atom i: NBG[g * n + i]
     int* NN = nnb + off;
     int* SL = slot_of + off;
-    const float Lf = (float)rep.L, invLf = (float)(1.0 / rep.L);
     const int nc = rep.nc;
     const int ncells = nc * nc * nc;
     const double dt = P.dt, dtsq = dt * dt;
@@ -202,7 +204,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                     XR0[i] = x0; XR1[i] = x1; XR2[i] = x2;
                     const float4 w = batch_wrap(x0, x1, x2, rep.L, rep.invL);
                     xr0[i] = w.x; xr1[i] = w.y; xr2[i] = w.z;
-                    const int c = batch_cell(w, rep.inv_cs, 0.5 * rep.L, nc);
+                    const int c = batch_cell(w, nc);
                     code[m] = c * BATCH_NMAX + atomicAdd(&ccount[c], 1);
                 }
             }
@@ -255,22 +257,22 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             for (int s = tid; s < n; s += BATCH_THREADS) {
                 const int i = tidx[s];
                 const float4 pi = spos[s];
-                const int c0 = batch_cell(pi, rep.inv_cs, 0.5 * rep.L, nc);
+                const int c0 = batch_cell(pi, nc);
                 const int cz = c0 / (nc * nc), cy = (c0 / nc) % nc, cx = c0 % nc;
                 int cnt = 0;
                 uint64_t lo = 0, hi = 0;
                 for (int dz = -1; dz <= 1; ++dz) {
                     int zz = cz + dz;
                     float oz = 0.0f;
-                    if (zz < 0) { zz += nc; oz = -Lf; } else if (zz >= nc) { zz -= nc; oz = Lf; }
+                    if (zz < 0) { zz += nc; oz = -1.0f; } else if (zz >= nc) { zz -= nc; oz = 1.0f; }
                     for (int dy = -1; dy <= 1; ++dy) {
                         int yy = cy + dy;
                         float oy = 0.0f;
-                        if (yy < 0) { yy += nc; oy = -Lf; } else if (yy >= nc) { yy -= nc; oy = Lf; }
+                        if (yy < 0) { yy += nc; oy = -1.0f; } else if (yy >= nc) { yy -= nc; oy = 1.0f; }
                         for (int dx = -1; dx <= 1; ++dx) {
                             int xx = cx + dx;
                             float ox = 0.0f;
-                            if (xx < 0) { xx += nc; ox = -Lf; } else if (xx >= nc) { xx -= nc; ox = Lf; }
+                            if (xx < 0) { xx += nc; ox = -1.0f; } else if (xx >= nc) { xx -= nc; ox = 1.0f; }
                             // the neighbour cell's atoms at their image next to this cell
                             const float qx = pi.x - ox, qy = pi.y - oy, qz = pi.z - oz;
                             const int c = (zz * nc + yy) * nc + xx;
@@ -290,7 +292,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                                         const float ex = qx - pj[t].x, ey = qy - pj[t].y,
                                                     ez = qz - pj[t].z;
                                         const bool hit = (k + t < nb) & (ub + k + t != s) &
-                                                         (ex * ex + ey * ey + ez * ez < P.rlist2f);
+                                                         (ex * ex + ey * ey + ez * ez < rep.rlist2);
                                         hits |= (unsigned)hit << (k + t);
                                     }
                                 }
@@ -322,10 +324,10 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 for (int j = 0; j < n; ++j) {
                     const float4 pj = spos[j];
                     float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    dx -= Lf * rintf(dx * invLf);
-                    dy -= Lf * rintf(dy * invLf);
-                    dz -= Lf * rintf(dz * invLf);
-                    if (j != i && dx * dx + dy * dy + dz * dz < P.rlist2f) emit(i, cnt, lo, hi, (unsigned)j);
+                    dx -= rintf(dx);
+                    dy -= rintf(dy);
+                    dz -= rintf(dz);
+                    if (j != i && dx * dx + dy * dy + dz * dz < rep.rlist2) emit(i, cnt, lo, hi, (unsigned)j);
                 }
                 finish(i, cnt, lo, hi);
             }
@@ -357,15 +359,16 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             // masked entry) is selected to zero and the products enter the group sums
             auto pair = [&](const float4 pj, bool ok, float& gx, float& gy, float& gz) {
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                dx -= Lf * rintf(dx * invLf);
-                dy -= Lf * rintf(dy * invLf);
-                dz -= Lf * rintf(dz * invLf);
+                dx -= rintf(dx);
+                dy -= rintf(dy);
+                dz -= rintf(dz);
+                // fractional units: with t = (L/r)^6, 2 (sigma/r)^12 - (sigma/r)^6 =
+                // t (c12 t - c6); the factor 24 eps / L is applied to the sums
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 const float inv = __builtin_amdgcn_rcpf(r2);
-                const float q = P.s2 * inv;
-                const float s6 = q * q * q;
-                float f = (2.0f * s6 * s6 - s6) * inv;
-                f = (ok && r2 < P.R2) ? f : 0.0f;
+                const float t = inv * inv * inv;
+                float f = inv * (t * fmaf(rep.c12, t, -rep.c6));
+                f = (ok && r2 < rep.R2) ? f : 0.0f;
                 gx = fmaf(f, dx, gx);
                 gy = fmaf(f, dy, gy);
                 gz = fmaf(f, dz, gz);
@@ -400,7 +403,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 cur = nxt;
             }
             if (nfull < ng) group(cur, cnt - 8 * nfull, std::true_type{});
-            const double c = P.c24eps / mi;
+            const double c = P.c24eps * rep.invL / mi;
             const double ax = c * fx.value(), ay = c * fy.value(), az = c * fz.value();
             A0[i] = ax;
             A1[i] = ay;
@@ -466,10 +469,10 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             const float4 w = batch_wrap(px, py, pz, rep.L, rep.invL);
             spos[d.sl] = w;
             float ex = w.x - xr0[i], ey = w.y - xr1[i], ez = w.z - xr2[i];
-            ex -= Lf * rintf(ex * invLf);
-            ey -= Lf * rintf(ey * invLf);
-            ez -= Lf * rintf(ez * invLf);
-            if (ex * ex + ey * ey + ez * ez > P.half_skin2f) rebuild = 1;
+            ex -= rintf(ex);
+            ey -= rintf(ey);
+            ez -= rintf(ez);
+            if (ex * ex + ey * ey + ez * ez > rep.hs2) rebuild = 1;
         };
         const int t0 = batch_lane();
         if (rep.therm == NBX_THERM_LANGEVIN) {
@@ -580,19 +583,25 @@ int batch_nmax() { return BATCH_NMAX; }
 
 // host-side fill of the replica records (layout private to this file)
 void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t therm, double L,
-                    double T, double param, double dt, double m0, uint64_t key, double rlist,
-                    double kb) {
+                    double T, double param, double dt, double m0, uint64_t key, double sigma,
+                    double R, double skin, double kb) {
+    const double rlist = R + skin;
     nbx_batch_rep r{};
     int nc = (int)floor(L / rlist);
     while (nc * nc * nc > BATCH_MAXCELLS) --nc;
     r.nc = nc >= 3 ? nc : 0;
-    r.inv_cs = r.nc ? (double)r.nc / L : 0.0;
     r.off = off;
     r.list_off = list_off;
     r.n = n;
     r.therm = therm;
     r.L = L;
     r.invL = 1.0 / L;
+    const double c6 = pow(sigma / L, 6.0);
+    r.c6 = (float)c6;
+    r.c12 = (float)(2.0 * c6 * c6);
+    r.R2 = (float)((R / L) * (R / L));
+    r.rlist2 = (float)((rlist / L) * (rlist / L)) * 1.0001f;
+    r.hs2 = (float)(0.25 * (skin / L) * (skin / L));
     r.T = T;
     r.param = param;
     r.m0 = m0;
@@ -610,12 +619,8 @@ void batch_fill_rep(void* dst, int64_t off, int64_t list_off, int32_t n, int32_t
 void batch_fill_params(void* dst, double eps, double sigma, double R, double rlist, double skin,
                        double dt, double kb, int cap, int64_t ntot) {
     nbx_batch_params P{};
-    P.s2 = (float)(sigma * sigma);
-    P.R2 = (float)(R * R);
-    P.rlist2f = (float)(rlist * rlist) * 1.0001f;
+    (void)sigma; (void)R; (void)rlist; (void)skin;
     P.c24eps = 24.0 * eps;
-    P.half_skin2 = 0.25 * skin * skin;
-    P.half_skin2f = (float)P.half_skin2;
     P.dt = dt;
     P.kb = kb;
     P.cap = cap;

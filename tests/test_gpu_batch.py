@@ -159,7 +159,14 @@ def test_batch_autotune_keeps_state_and_is_reproducible(nbx):
 def test_batch_list_overflow_replays_the_call(nbx):
     # imploding dilute replicas: the per-atom list capacity sized for the initial density
     # is exceeded inside the launch; the call is restored from its device backup and
-    # replayed with larger lists.  Every replica must follow the single-system path.
+    # replayed with larger lists.  The replay is deterministic (two batches agree bit for
+    # bit) and every replica follows the single-system path.  Bound of the latter: the
+    # truncated LJ force jumps by |F(R)| = 0.039 at the cutoff, and the two paths round r^2
+    # differently (box-unit FP32 coordinates against FP32 coordinates in sigma), so a pair
+    # that sits at R within rounding may count in one path one step earlier: a velocity
+    # difference of up to |F(R)| dt / m = 7.8e-5, i.e. up to 7.8e-5 in position per such
+    # event over the remaining t <= 1.  A replay that lost or repeated steps, or kept
+    # truncated lists, is off by 1e-3 or more.
     print("[case] batch implosion", flush=True)
     systems = [gen.lj_liquid(c, rho=0.25, T=0.1, seed=5 + c, jitter=0.05) for c in (5, 6)]
     vs = []
@@ -167,13 +174,17 @@ def test_batch_list_overflow_replays_the_call(nbx):
         centre = 0.5 * s.L
         vs.append(s.v - 2.0 * (s.u - centre) / centre)
     natoms = [s.u.shape[0] for s in systems]
-    b = _lib.Batch(natoms, [s.L for s in systems], np.concatenate([s.masses for s in systems]),
-                   1.0, 1.0, 2.5, 1.0)
-    b.set_state(np.concatenate([s.u for s in systems]), np.concatenate(vs))
-    slots0 = b.stats()["list_slots"]
-    b.run(0.002, 500)
-    slots1 = b.stats()["list_slots"]
-    Ub, _ = b.get_state()
+    runs = []
+    for _ in range(2):
+        b = _lib.Batch(natoms, [s.L for s in systems],
+                       np.concatenate([s.masses for s in systems]), 1.0, 1.0, 2.5, 1.0)
+        b.set_state(np.concatenate([s.u for s in systems]), np.concatenate(vs))
+        slots0 = b.stats()["list_slots"]
+        b.run(0.002, 500)
+        slots1 = b.stats()["list_slots"]
+        runs.append(b.get_state())
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    Ub = runs[0][0]
     o, devs = 0, []
     for s, v in zip(systems, vs):
         n = s.u.shape[0]
@@ -190,4 +201,4 @@ def test_batch_list_overflow_replays_the_call(nbx):
     print(f"[case] batch implosion: list slots {slots0} -> {slots1}, max |du| {max(devs):.2e}",
           flush=True)
     assert slots1 > slots0
-    assert max(devs) < 2e-5
+    assert max(devs) < 4e-4
