@@ -137,6 +137,7 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
     __shared__ uint16_t tidx[BATCH_NMAX];         // atom indices by slot during a build
     __shared__ int wsum[BATCH_THREADS / 64];      // wave totals of the cell-count scan
     __shared__ int rebuild;
+    __shared__ uint4 ebuf[BATCH_THREADS];         // list entries being packed, per thread
     // reference positions of the last build (FP32, wrapped and centred), by atom
     __shared__ float xr0[BATCH_NMAX], xr1[BATCH_NMAX], xr2[BATCH_NMAX];
     const nbx_batch_rep rep = reps[order[blockIdx.x]];
@@ -166,29 +167,24 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
     // each atom among its cell's atoms by index (so slots and lists are reproducible),
     // then every slot scans the 27 neighbour cells with one periodic shift per cell.
     // Otherwise slots are atom indices and every atom scans all atoms.  List entries are
-    // packed 8 to a 16-byte group in registers and written a group at a time.
+    // packed 8 to a 16-byte group in an LDS staging slot of the thread and written a group
+    // at a time.
     auto build = [&]() {
         const int tid = batch_lane();
         uint4* NBW = reinterpret_cast<uint4*>(NB);
         // entry `cnt` of atom i is slot u; a full group (or the final partial one) is stored
-        auto emit = [&](int i, int& cnt, uint64_t& lo, uint64_t& hi, unsigned u) {
+        // the thread's group of 8 entries is staged in LDS (entries of a partial final group
+        // past the count keep older values: the pair loop never reads them)
+        uint16_t* const eb = reinterpret_cast<uint16_t*>(&ebuf[tid]);
+        auto emit = [&](int i, int& cnt, unsigned u) {
             const int e = cnt & 7;
-            const uint64_t val = (uint64_t)u << (16 * (e & 3));
-            lo |= e < 4 ? val : 0;
-            hi |= e < 4 ? 0 : val;
-            if (e == 7) {
-                if (cnt < P.cap)
-                    NBW[(int64_t)(cnt >> 3) * n + i] =
-                        make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
-                lo = 0; hi = 0;
-            }
+            eb[e] = (uint16_t)u;
+            if (e == 7 && cnt < P.cap) NBW[(int64_t)(cnt >> 3) * n + i] = ebuf[tid];
             ++cnt;
         };
-        auto finish = [&](int i, int cnt, uint64_t lo, uint64_t hi) {
+        auto finish = [&](int i, int cnt) {
             if (cnt > P.cap) { flags[0] = 1; cnt = P.cap; }
-            else if (cnt & 7)
-                NBW[(int64_t)(cnt >> 3) * n + i] =
-                    make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+            else if (cnt & 7) NBW[(int64_t)(cnt >> 3) * n + i] = ebuf[tid];
             NN[i] = cnt;
         };
         if (nc >= 3) {
@@ -260,7 +256,6 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                 const int c0 = batch_cell(pi, nc);
                 const int cz = c0 / (nc * nc), cy = (c0 / nc) % nc, cx = c0 % nc;
                 int cnt = 0;
-                uint64_t lo = 0, hi = 0;
                 for (int dz = -1; dz <= 1; ++dz) {
                     int zz = cz + dz;
                     float oz = 0.0f;
@@ -278,34 +273,40 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
                             const int c = (zz * nc + yy) * nc + xx;
                             const int u1 = cstart[c + 1];
                             // hits of up to 32 candidates as a bit mask (positions read 4 at a
-                            // time ahead of the tests), then the set bits in order
+                            // time ahead of the tests, then the last 0..3), the atom's own slot
+                            // cleared from the mask, then the set bits in order
                             for (int ub = cstart[c]; ub < u1; ub += 32) {
                                 const int nb = u1 - ub < 32 ? u1 - ub : 32;
                                 unsigned hits = 0;
-                                for (int k = 0; k < nb; k += 4) {
+                                int k = 0;
+                                for (; k + 4 <= nb; k += 4) {
                                     float4 pj[4];
 #pragma unroll
-                                    for (int t = 0; t < 4; ++t)
-                                        pj[t] = spos[k + t < nb ? ub + k + t : ub];
+                                    for (int t = 0; t < 4; ++t) pj[t] = spos[ub + k + t];
 #pragma unroll
                                     for (int t = 0; t < 4; ++t) {
                                         const float ex = qx - pj[t].x, ey = qy - pj[t].y,
                                                     ez = qz - pj[t].z;
-                                        const bool hit = (k + t < nb) & (ub + k + t != s) &
-                                                         (ex * ex + ey * ey + ez * ez < rep.rlist2);
-                                        hits |= (unsigned)hit << (k + t);
+                                        hits |= (unsigned)(ex * ex + ey * ey + ez * ez < rep.rlist2)
+                                                << (k + t);
                                     }
                                 }
+                                for (; k < nb; ++k) {
+                                    const float4 pj = spos[ub + k];
+                                    const float ex = qx - pj.x, ey = qy - pj.y, ez = qz - pj.z;
+                                    hits |= (unsigned)(ex * ex + ey * ey + ez * ez < rep.rlist2) << k;
+                                }
+                                if ((unsigned)(s - ub) < 32u) hits &= ~(1u << (s - ub));
                                 while (hits) {
                                     const int b = __builtin_ctz(hits);
                                     hits &= hits - 1;
-                                    emit(i, cnt, lo, hi, (unsigned)(ub + b));
+                                    emit(i, cnt, (unsigned)(ub + b));
                                 }
                             }
                         }
                     }
                 }
-                finish(i, cnt, lo, hi);
+                finish(i, cnt);
             }
         } else {
             for (int i = tid; i < n; i += BATCH_THREADS) {
@@ -320,16 +321,15 @@ k_batch_steps(const nbx_batch_rep* __restrict__ reps, const int* __restrict__ or
             for (int i = tid; i < n; i += BATCH_THREADS) {
                 const float4 pi = spos[i];
                 int cnt = 0;
-                uint64_t lo = 0, hi = 0;
                 for (int j = 0; j < n; ++j) {
                     const float4 pj = spos[j];
                     float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     dx -= rintf(dx);
                     dy -= rintf(dy);
                     dz -= rintf(dz);
-                    if (j != i && dx * dx + dy * dy + dz * dz < rep.rlist2) emit(i, cnt, lo, hi, (unsigned)j);
+                    if (j != i && dx * dx + dy * dy + dz * dz < rep.rlist2) emit(i, cnt, (unsigned)j);
                 }
-                finish(i, cnt, lo, hi);
+                finish(i, cnt);
             }
         }
         if (tid == 0) atomicAdd(&flags[1], 1);
