@@ -403,11 +403,14 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
 //      their current positions into the next bin set);
 //   2. the atoms of the stencil cells -- the 27 neighbour cells, or with rotate the 125
 //      cells within two half-size cells (fixed order; in batches of <= 1024) -- are
-//      staged in LDS -- indices, then FP32 positions, every load independent -- and
-//      rank-sorted by index within each cell, so every list comes out in a reproducible
-//      order (cell order, then index order);
+//      staged in LDS -- indices, then FP32 positions (with per-step bins: at the periodic
+//      image of their cell next to the home cell), every load independent -- and sorted
+//      by index within each cell, so every list comes out in a reproducible order (cell
+//      order, then index order);
 //   3. wave w takes own atoms w, w+4, ...; for each, its 64 lanes test 64 candidates at
-//      a time and append the hits in candidate order (ballot + prefix count).
+//      a time (per-step bins: a plain difference, the images being the near ones; other
+//      bins: the minimum image) and append the hits in candidate order (ballot + prefix
+//      count).
 //   The squared list radius carries a 1e-4 relative margin so that the FP32 test never
 //   drops a pair that the FP64 distance puts inside the list radius.
 // Per-step bins: every block also clears its cell in the other bin set (used by the next
@@ -434,25 +437,26 @@ __device__ __forceinline__ int build_segment(const int* seg, int nseg, int t) {
 
 #define BUILD_STENCIL 125
 
-template <bool WATER>
+template <bool WATER, bool SHIFT>
 __global__ void __launch_bounds__(BUILD_THREADS)
 k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int n, int nc,
-              float Lf, float invLf, int* __restrict__ bin_count, int* __restrict__ bin_atoms,
+              double Ld, int* __restrict__ bin_count, int* __restrict__ bin_atoms,
               int ccap, double inv_cs, float rlist2f, int G, int cap,
               int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
               int* __restrict__ flags, int* __restrict__ req, int clear_bins, int rotate,
               int parts) {
     __shared__ int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
-    __shared__ int sord[BUILD_SLOTS];
     __shared__ unsigned char sseg[BUILD_SLOTS];
     __shared__ int seg[BUILD_STENCIL + 1], segcell[BUILD_STENCIL];
     __shared__ int ncnt[BUILD_STENCIL], ncell[BUILD_STENCIL];
+    __shared__ float4 nshift[BUILD_STENCIL], segshift[BUILD_STENCIL];
     __shared__ int okey[BUILD_OWN], ocnt[BUILD_OWN];
     __shared__ float4 opos[BUILD_OWN];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const float Lf = (float)Ld, invLf = (float)(1.0 / Ld);
     // `parts` blocks share a home cell, each taking a contiguous slice of its atoms (more
     // blocks in flight when the cells are few and large, as for water)
     const int cid = blockIdx.x / parts, part = blockIdx.x % parts;
@@ -477,6 +481,10 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
         const int m2 = count[nid];
         ncell[tid] = nid;
         ncnt[tid] = m2 < ccap ? m2 : ccap;
+        // the periodic image of the stencil cell next to the home cell: -1, 0 or +1 box
+        // sides per axis
+        const auto img = [nc](int c) { return c < 0 ? -1.0f : (c >= nc ? 1.0f : 0.0f); };
+        nshift[tid] = make_float4(img(cx + ox), img(cy + oy), img(cz + oz), 0.0f);
     }
     int m = count[cid];
     m = m < ccap ? m : ccap;
@@ -509,7 +517,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             while (nb < nst) {
                 const int m2 = ncnt[nb];
                 if (M + m2 > BUILD_SLOTS && nseg > 0) break;
-                if (tid == 0) { seg[nseg] = M; segcell[nseg] = ncell[nb]; }
+                if (tid == 0) { seg[nseg] = M; segcell[nseg] = ncell[nb]; segshift[nseg] = nshift[nb]; }
                 M += m2;
                 ++nseg;
                 ++nb;
@@ -522,15 +530,37 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                 skey[t] = atoms[(size_t)segcell[s] * ccap + (t - seg[s])];
             }
             __syncthreads();
-            for (int t = tid; t < M; t += BUILD_THREADS) {
-                const double4 p = pos[skey[t]];
-                spos[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
-                const int s = sseg[t];
-                const int b = seg[s], e1 = seg[s + 1];
-                const int key = skey[t];
-                int r = 0;
-                for (int u = b; u < e1; ++u) r += skey[u] < key;
-                sord[b + r] = t;   // rank within its cell
+            // positions of the candidates at their image next to the home cell (FP64 sum,
+            // one FP32 rounding) and their rank by index within their cell; then both are
+            // stored in that order, so the tests below read candidates in list order
+            float4 cp[BUILD_SLOTS / BUILD_THREADS];
+            int ck[BUILD_SLOTS / BUILD_THREADS], cr[BUILD_SLOTS / BUILD_THREADS];
+#pragma unroll
+            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+                const int t = tid + q * BUILD_THREADS;
+                ck[q] = -1;
+                if (t < M) {
+                    const int key = skey[t];
+                    const double4 p = pos[key];
+                    const int s = sseg[t];
+                    const int b = seg[s], e1 = seg[s + 1];
+                    int r = 0;
+                    for (int u = b; u < e1; ++u) r += skey[u] < key;
+                    const float4 sh = SHIFT ? segshift[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    cp[q] = make_float4((float)(p.x + (double)sh.x * Ld),
+                                        (float)(p.y + (double)sh.y * Ld),
+                                        (float)(p.z + (double)sh.z * Ld), 0.0f);
+                    ck[q] = key;
+                    cr[q] = b + r;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+                if (ck[q] >= 0) {
+                    skey[cr[q]] = ck[q];
+                    spos[cr[q]] = cp[q];
+                }
             }
             __syncthreads();
             for (int o = wave; o < mo; o += BUILD_THREADS / 64) {
@@ -542,13 +572,14 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                     bool hit = false;
                     int j = 0;
                     if (t < M) {
-                        const int e = sord[t];
-                        j = skey[e];
-                        const float4 pj = spos[e];
+                        j = skey[t];
+                        const float4 pj = spos[t];
                         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                        dx -= Lf * rintf(dx * invLf);
-                        dy -= Lf * rintf(dy * invLf);
-                        dz -= Lf * rintf(dz * invLf);
+                        if (!SHIFT) {
+                            dx -= Lf * rintf(dx * invLf);
+                            dy -= Lf * rintf(dy * invLf);
+                            dz -= Lf * rintf(dz * invLf);
+                        }
                         hit = dx * dx + dy * dy + dz * dz < rlist2f && j != i &&
                               !(WATER && (j / 3) == (i / 3));
                     }
@@ -577,16 +608,15 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     }
     // the last block to finish (every block has read the request, the set index and its
     // bins) clears the request and, with per-step bins, the counts of the bins it used;
-    // rotate: advances the set
+    // rotate: advances the set.  No fence: the last block reads nothing that the other
+    // blocks wrote (the device-scope counter aside), and every block's reads of what it
+    // clears returned before its barrier (their values are in LDS); a device-scope fence
+    // here would write back the L2 of the block's XCD once per cell
     __shared__ int last;
     __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        last = atomicAdd(reinterpret_cast<unsigned*>(&state[1]), 1u) == gridDim.x - 1;
-    }
+    if (tid == 0) last = atomicAdd(reinterpret_cast<unsigned*>(&state[1]), 1u) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
-    __threadfence();
     if (clear_bins)
         for (int c = tid; c < ncells; c += BUILD_THREADS) bin_count[c] = 0;
     if (tid == 0) {
@@ -640,11 +670,9 @@ __global__ void k_build_allpairs(const double4* __restrict__ pos, const double* 
     if (i < n) build_allpairs_atom<WATER>(pos, x, i, n, L, invL, rlist2, G, cap, nbr, nnb, xref, flags);
     __shared__ int last;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
+    if (threadIdx.x == 0)   // no fence: the last block only clears what every block read
         last = atomicAdd(reinterpret_cast<unsigned*>(&flags[NBX_FLAG_BINSET + 1]), 1u) ==
                gridDim.x - 1;
-    }
     __syncthreads();
     if (last && threadIdx.x == 0) {
         flags[NBX_FLAG_BINSET + 1] = 0;
@@ -669,10 +697,20 @@ static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, con
         int parts = (512 + ncells - 1) / ncells;
         parts = parts < 1 ? 1 : (parts > 8 ? 8 : parts);
         const dim3 grid(ncells * parts);
-        hipLaunchKernelGGL((k_build_cells<WATER>), grid, dim3(BUILD_THREADS), 0, s,
-                           (const double4*)pos, x, n, nc, (float)L, (float)(1.0 / L), bin_count,
-                           bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
-                           nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
+        // per-step bins hold every atom in the cell of its current position, so a stencil
+        // cell's atoms are shifted once to their image next to the home cell; bins made
+        // earlier (set-up bins of 3 cells per side, bins of the previous build) need the
+        // minimum image per candidate
+        if (nl.bin_mode == NBX_BINS_STEP)
+            hipLaunchKernelGGL((k_build_cells<WATER, true>), grid, dim3(BUILD_THREADS), 0, s,
+                               (const double4*)pos, x, n, nc, L, bin_count,
+                               bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
+                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
+        else
+            hipLaunchKernelGGL((k_build_cells<WATER, false>), grid, dim3(BUILD_THREADS), 0, s,
+                               (const double4*)pos, x, n, nc, L, bin_count,
+                               bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
+                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
     } else {
         hipLaunchKernelGGL((k_build_allpairs<WATER>), dim3(nbx_blocks(n, 128)), dim3(128), 0,
                            s, (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap, nbr,
