@@ -407,10 +407,10 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
 //      image of their cell next to the home cell), every load independent -- and sorted
 //      by index within each cell, so every list comes out in a reproducible order (cell
 //      order, then index order);
-//   3. wave w takes own atoms w, w+4, ...; for each, its 64 lanes test 64 candidates at
-//      a time (per-step bins: a plain difference, the images being the near ones; other
-//      bins: the minimum image) and append the hits in candidate order (ballot + prefix
-//      count).
+//   3. wave w takes own atoms 8 at a time; its 64 lanes read 64 candidates at a time and
+//      test each against the 8 (per-step bins: a plain difference, the images being the
+//      near ones; other bins: the minimum image), appending every atom's hits in candidate
+//      order (ballot + prefix count).
 //   The squared list radius carries a 1e-4 relative margin so that the FP32 test never
 //   drops a pair that the FP64 distance puts inside the list radius.
 // Per-step bins: every block also clears its cell in the other bin set (used by the next
@@ -424,6 +424,7 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
 #define BUILD_THREADS 256
 #define BUILD_SLOTS 1024
 #define BUILD_OWN 256
+#define BUILD_OWN_BATCH 8
 
 __device__ __forceinline__ int build_segment(const int* seg, int nseg, int t) {
     int lo = 0, hi = nseg - 1;   // last s with seg[s] <= t
@@ -563,34 +564,51 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                 }
             }
             __syncthreads();
-            for (int o = wave; o < mo; o += BUILD_THREADS / 64) {
-                const int i = okey[o];
-                const float4 pi = opos[o];
-                int c = ocnt[o];
+            // wave w tests the candidates against its own atoms 8 at a time (own atoms
+            // 8w .. 8w+7, then 8w+32 ..): each candidate is read from LDS once per 8 tests,
+            // and the 8 tests of a candidate are independent
+            for (int o0 = wave * BUILD_OWN_BATCH; o0 < mo; o0 += BUILD_THREADS / 64 * BUILD_OWN_BATCH) {
+                const int no = min(BUILD_OWN_BATCH, mo - o0);
+                float4 pi[BUILD_OWN_BATCH];
+                int ii[BUILD_OWN_BATCH], cc[BUILD_OWN_BATCH];
+#pragma unroll
+                for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
+                    const int o = k < no ? o0 + k : o0;
+                    pi[k] = opos[o];
+                    ii[k] = okey[o];
+                    cc[k] = ocnt[o];
+                }
                 for (int tb = 0; tb < M; tb += 64) {
                     const int t = tb + lane;
-                    bool hit = false;
-                    int j = 0;
-                    if (t < M) {
-                        j = skey[t];
-                        const float4 pj = spos[t];
-                        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const bool v = t < M;
+                    const int j = skey[v ? t : 0];
+                    const float4 pj = spos[v ? t : 0];
+                    const int jm = WATER ? j / 3 : 0;
+#pragma unroll
+                    for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
+                        if (k >= no) break;   // uniform over the wave
+                        float dx = pi[k].x - pj.x, dy = pi[k].y - pj.y, dz = pi[k].z - pj.z;
                         if (!SHIFT) {
                             dx -= Lf * rintf(dx * invLf);
                             dy -= Lf * rintf(dy * invLf);
                             dz -= Lf * rintf(dz * invLf);
                         }
-                        hit = dx * dx + dy * dy + dz * dz < rlist2f && j != i &&
-                              !(WATER && (j / 3) == (i / 3));
+                        const bool hit = v && dx * dx + dy * dy + dz * dz < rlist2f && j != ii[k] &&
+                                         !(WATER && jm == ii[k] / 3);
+                        const unsigned long long mask = __ballot(hit);
+                        if (hit) {
+                            const int r = cc[k] + __popcll(mask & below);
+                            if (r < cap)
+                                nbr[(size_t)(r / G) * stride + (size_t)ii[k] * G + (r % G)] = j;
+                        }
+                        cc[k] += __popcll(mask);
                     }
-                    const unsigned long long mask = __ballot(hit);
-                    if (hit) {
-                        const int r = c + __popcll(mask & below);
-                        if (r < cap) nbr[(size_t)(r / G) * stride + (size_t)i * G + (r % G)] = j;
-                    }
-                    c += __popcll(mask);
                 }
-                if (lane == 0) ocnt[o] = c;
+                if (lane == 0) {
+#pragma unroll
+                    for (int k = 0; k < BUILD_OWN_BATCH; ++k)
+                        if (k < no) ocnt[o0 + k] = cc[k];
+                }
             }
             __syncthreads();
         }
