@@ -452,6 +452,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     __shared__ int seg[BUILD_STENCIL + 1], segcell[BUILD_STENCIL];
     __shared__ int ncnt[BUILD_STENCIL], ncell[BUILD_STENCIL];
     __shared__ float4 nshift[BUILD_STENCIL], segshift[BUILD_STENCIL];
+    __shared__ int npre[BUILD_STENCIL + 1];   // exclusive prefix sums of ncnt
     __shared__ int okey[BUILD_OWN], ocnt[BUILD_OWN];
     __shared__ float4 opos[BUILD_OWN];
     const int tid = threadIdx.x;
@@ -493,6 +494,19 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     const int hb = part * per, he = (hb + per) < m ? (hb + per) : m;
     if (hb < he) {   // uniform over the block
     __syncthreads();
+    if (wave == 0) {   // prefix sums of the stencil counts (nst <= 128: two per lane)
+        const int a0 = 2 * lane < nst ? ncnt[2 * lane] : 0;
+        const int a1 = 2 * lane + 1 < nst ? ncnt[2 * lane + 1] : 0;
+        int v = a0 + a1;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(v, d, 64);
+            if (lane >= d) v += u;
+        }
+        const int ex = v - a0 - a1;
+        if (2 * lane <= nst) npre[2 * lane] = ex;
+        if (2 * lane + 1 <= nst) npre[2 * lane + 1] = ex + a0;
+    }
+    __syncthreads();
     const size_t stride = (size_t)n * G;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int own0 = hb; own0 < he; own0 += BUILD_OWN) {
@@ -513,17 +527,23 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
         }
         int nb = 0;
         while (nb < nst) {
-            // a batch of consecutive stencil cells that fits in LDS
-            int M = 0, nseg = 0;
-            while (nb < nst) {
-                const int m2 = ncnt[nb];
-                if (M + m2 > BUILD_SLOTS && nseg > 0) break;
-                if (tid == 0) { seg[nseg] = M; segcell[nseg] = ncell[nb]; segshift[nseg] = nshift[nb]; }
-                M += m2;
-                ++nseg;
-                ++nb;
+            // a batch of consecutive stencil cells that fits in LDS: the most cells from nb
+            // on whose counts sum to at most BUILD_SLOTS (a cell holds at most ccap <=
+            // BUILD_SLOTS atoms), found by bisection on the prefix sums
+            int lo = nb + 1, hi = nst;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (npre[mid] - npre[nb] <= BUILD_SLOTS) lo = mid;
+                else hi = mid - 1;
+            }
+            const int nseg = lo - nb, M = npre[lo] - npre[nb];
+            if (tid < nseg) {
+                seg[tid] = npre[nb + tid] - npre[nb];
+                segcell[tid] = ncell[nb + tid];
+                segshift[tid] = nshift[nb + tid];
             }
             if (tid == 0) seg[nseg] = M;
+            nb = lo;
             __syncthreads();
             for (int t = tid; t < M; t += BUILD_THREADS) {
                 const int s = build_segment(seg, nseg, t);
@@ -536,22 +556,32 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             // stored in that order, so the tests below read candidates in list order
             float4 cp[BUILD_SLOTS / BUILD_THREADS];
             int ck[BUILD_SLOTS / BUILD_THREADS], cr[BUILD_SLOTS / BUILD_THREADS];
+            float4 csh[BUILD_SLOTS / BUILD_THREADS];
 #pragma unroll
             for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
                 const int t = tid + q * BUILD_THREADS;
-                ck[q] = -1;
+                ck[q] = t < M ? skey[t] : -1;
+                csh[q] = (SHIFT && t < M) ? segshift[sseg[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            // the gathers of all four candidates are in flight together
+#pragma unroll
+            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+                if (ck[q] >= 0) {
+                    const double4 p = pos[ck[q]];
+                    cp[q] = make_float4((float)(p.x + (double)csh[q].x * Ld),
+                                        (float)(p.y + (double)csh[q].y * Ld),
+                                        (float)(p.z + (double)csh[q].z * Ld), 0.0f);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+                const int t = tid + q * BUILD_THREADS;
                 if (t < M) {
-                    const int key = skey[t];
-                    const double4 p = pos[key];
+                    const int key = ck[q];
                     const int s = sseg[t];
                     const int b = seg[s], e1 = seg[s + 1];
                     int r = 0;
                     for (int u = b; u < e1; ++u) r += skey[u] < key;
-                    const float4 sh = SHIFT ? segshift[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    cp[q] = make_float4((float)(p.x + (double)sh.x * Ld),
-                                        (float)(p.y + (double)sh.y * Ld),
-                                        (float)(p.z + (double)sh.z * Ld), 0.0f);
-                    ck[q] = key;
                     cr[q] = b + r;
                 }
             }
