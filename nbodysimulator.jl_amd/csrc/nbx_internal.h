@@ -46,7 +46,7 @@ struct nbx_nlist {
     int* d_cell_atoms = nullptr;  // 3 x ncells x ccap
     double* d_xref = nullptr; // unwrapped positions at the last build (SoA)
     int* d_flags = nullptr;   // [0] rebuild request, [1] overflow, [2] builds done,
-                              // [4] reduction counter, [5] current bin set, [6] build
+                              // [4] block count of the temperature partials, [5] current bin set, [6] build
                               // finish counter, [7] a list went stale between two build
                               // opportunities
     size_t cap_bytes = 0;

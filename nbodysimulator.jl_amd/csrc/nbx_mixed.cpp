@@ -289,7 +289,7 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
     io.need_T = ((c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER) && do_drift)
                     ? 1 : 0;
     io.red_part = c->d_red;
-    io.red_counter = reinterpret_cast<unsigned*>(c->nl.d_flags + 4);
+    io.red_count = c->nl.d_flags + 4;
     io.T_out = c->d_scal;
     io.T_denom = c->kb * (double)(3 * c->n - constraints(c));
     io.nh = nh_params(c, dt, 1);   // with need_T: the Nose-Hoover update of this drift
@@ -336,6 +336,7 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
         if (e1) (void)hipEventRecord(e1, s);
         if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
     }
+    if (io.need_T) mx_launch_temperature_finish(s, io);
     if (do_drift && next_opp) after_drift(c, p, par ^ 1);
 }
 
@@ -568,7 +569,9 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 kp.step_base = c->d_step;
                 kp.step_off = 0;
                 const int first_opp = (step0 % build_every(c)) == 0 ? 1 : 0;
-                mx_launch_drift(s, c, kp, make_io(c, p, 0, 1, dt, first_opp));
+                const mx_io io0 = make_io(c, p, 0, 1, dt, first_opp);
+                mx_launch_drift(s, c, kp, io0);
+                if (io0.need_T) mx_launch_temperature_finish(s, io0);
                 if (first_opp) after_drift(c, p, 1);
             }
             const int gs = c->graph_steps;

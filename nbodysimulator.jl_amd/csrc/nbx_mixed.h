@@ -83,7 +83,7 @@ struct mx_io {
     // Berendsen temperature of vh_{k+1}
     int need_T;
     double* red_part;     // one partial per block
-    unsigned* red_counter;
+    int* red_count;       // number of partials of the last step kernel
     double* T_out;
     double T_denom;       // kb (3N - Nc)
     mx_nh_params nh;      // Nose-Hoover update applied with the temperature
@@ -263,15 +263,17 @@ __device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
 }
 
 // Epilogue of a step kernel, called by every thread of every block (blockDim.x <= 256):
-// with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x]; the last
-// block to finish (device-scope counter) adds the partials in block order, writes the
-// temperature of vh_{k+1} and applies the Nose-Hoover update.  Deterministic: the final
-// sum does not depend on which block finishes last.
+// with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x] and block 0
+// records the number of blocks in *red_count.  mx_launch_temperature_finish, launched next
+// on the stream, adds the partials in block order, writes the temperature of vh_{k+1} and
+// applies the Nose-Hoover update.  A separate launch instead of a last-block reduction:
+// the kernel boundary makes every partial visible, where a device-scope fence per block
+// (writing back its XCD's L2) cost more than the launch.  Deterministic: the sum does not
+// depend on the order in which blocks finish.
 __device__ __forceinline__ void mx_epilogue(double val, const mx_io& io, int n) {
     (void)n;
     if (!io.need_T) return;
     __shared__ double red[256];
-    __shared__ int last;
     const int t = threadIdx.x;
     const int nt = (int)blockDim.x;
     red[t] = val;
@@ -282,31 +284,7 @@ __device__ __forceinline__ void mx_epilogue(double val, const mx_io& io, int n) 
     }
     if (t == 0) {
         io.red_part[blockIdx.x] = red[0];
-        __threadfence();
-        last = atomicAdd(io.red_counter, 1u) == gridDim.x - 1u;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double s = 0.0;
-    for (int b = t; b < (int)gridDim.x; b += nt) s += __builtin_nontemporal_load(io.red_part + b);
-    red[t] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off && t + off < nt) red[t] += red[t + off];
-        __syncthreads();
-    }
-    if (t == 0) {
-        const double T = red[0] / io.T_denom;
-        io.T_out[0] = T;
-        if (io.nh.nh) {
-            if (io.nh.advance)
-                io.T_out[NBX_SCAL_ZETA] = io.T_out[NBX_SCAL_ZETA] + io.nh.dt * io.T_out[NBX_SCAL_XI];
-            const double it = 1.0 / io.nh.tau;
-            io.T_out[NBX_SCAL_XI] =
-                (it * it) * (T / io.nh.T0 - (double)(io.nh.ndf + 1) / (double)io.nh.ndf);
-        }
-        *io.red_counter = 0u;   // ready for the next launch
+        if (blockIdx.x == 0) *io.red_count = (int)gridDim.x;
     }
 }
 
@@ -327,6 +305,9 @@ void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_pa
 // start of step kp.step + 1 from (x_in, v, a): drift of every atom
 void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
                      const mx_io& io);
+// with io.need_T: the temperature of vh_{k+1} from the block partials of the step kernel
+// just launched (and the Nose-Hoover update)
+void mx_launch_temperature_finish(hipStream_t s, const mx_io& io);
 // Verlet list (gated by req) from bins B; its last block clears req and, with
 // clear_bins, the counts of B (NBX_BINS_BUILD: B is the base of the three rotating sets,
 // which the build advances)

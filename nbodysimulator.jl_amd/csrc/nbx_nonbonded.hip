@@ -397,6 +397,38 @@ void mx_launch_nh_rate(hipStream_t s, double* scal, const mx_nh_params& nh) {
     hipLaunchKernelGGL(k_nh_rate, dim3(1), dim3(64), 0, s, scal, nh);
 }
 
+// the temperature of vh_{k+1}: the block partials of the step kernel before it on the
+// stream, added in block order (each thread a strided run of blocks, then a fixed tree),
+// and the Nose-Hoover update (src/thermostats.jl:116-128)
+__global__ void __launch_bounds__(256) k_temperature_finish(mx_io io) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int nb = *io.red_count;
+    double s = 0.0;
+    for (int b = t; b < nb; b += 256) s += io.red_part[b];
+    red[t] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double T = red[0] / io.T_denom;
+        io.T_out[0] = T;
+        if (io.nh.nh) {
+            if (io.nh.advance)
+                io.T_out[NBX_SCAL_ZETA] = io.T_out[NBX_SCAL_ZETA] + io.nh.dt * io.T_out[NBX_SCAL_XI];
+            const double it = 1.0 / io.nh.tau;
+            io.T_out[NBX_SCAL_XI] =
+                (it * it) * (T / io.nh.T0 - (double)(io.nh.ndf + 1) / (double)io.nh.ndf);
+        }
+    }
+}
+
+void mx_launch_temperature_finish(hipStream_t s, const mx_io& io) {
+    hipLaunchKernelGGL(k_temperature_finish, dim3(1), dim3(256), 0, s, io);
+}
+
 // ---------------------------------------------------------------------------
 // Verlet list build (gated by *req): one block of 4 wavefronts per cell.
 //   1. the cell's own atoms (<= 256 per pass) are staged in LDS (rotate: and binned by
