@@ -626,18 +626,21 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                 }
             }
             __syncthreads();
-            // wave w tests the candidates against its own atoms 8 at a time (own atoms
-            // 8w .. 8w+7, then 8w+32 ..): each candidate is read from LDS once per 8 tests,
-            // and the 8 tests of a candidate are independent
-            for (int o0 = wave * BUILD_OWN_BATCH; o0 < mo; o0 += BUILD_THREADS / 64 * BUILD_OWN_BATCH) {
-                const int no = min(BUILD_OWN_BATCH, mo - o0);
+            // wave w tests the candidates against `per` of its own atoms at a time (own
+            // atoms per*w .. per*w+per-1, then per*(w+4) ..; per = ceil(own / 4) up to 8, so
+            // every wave has work): each candidate is read from LDS once per `per` tests,
+            // and those tests are independent
+            const int per = min(BUILD_OWN_BATCH, (mo + BUILD_THREADS / 64 - 1) / (BUILD_THREADS / 64));
+            for (int o0 = wave * per; o0 < mo; o0 += BUILD_THREADS / 64 * per) {
+                const int no = min(per, mo - o0);
                 float4 pi[BUILD_OWN_BATCH];
-                int ii[BUILD_OWN_BATCH], cc[BUILD_OWN_BATCH];
+                int ii[BUILD_OWN_BATCH], im[BUILD_OWN_BATCH], cc[BUILD_OWN_BATCH];
 #pragma unroll
                 for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
                     const int o = k < no ? o0 + k : o0;
                     pi[k] = opos[o];
                     ii[k] = okey[o];
+                    im[k] = WATER ? ii[k] / 3 : 0;
                     cc[k] = ocnt[o];
                 }
                 for (int tb = 0; tb < M; tb += 64) {
@@ -656,7 +659,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                             dz -= Lf * rintf(dz * invLf);
                         }
                         const bool hit = v && dx * dx + dy * dy + dz * dz < rlist2f && j != ii[k] &&
-                                         !(WATER && jm == ii[k] / 3);
+                                         !(WATER && jm == im[k]);
                         const unsigned long long mask = __ballot(hit);
                         if (hit) {
                             const int r = cc[k] + __popcll(mask & below);
