@@ -45,6 +45,10 @@ static void release(nbx_ctx* c) {
     free_ptr(c->d_scal);
     free_ptr(c->d_mm);
     free_ptr(c->d_mm4);
+    if (c->d_perm) (void)hipFree(c->d_perm);
+    if (c->d_iperm) (void)hipFree(c->d_iperm);
+    if (c->d_pmap) (void)hipFree(c->d_pmap);
+    c->d_perm = c->d_iperm = c->d_pmap = nullptr;
     c->d_mm = nullptr;
     c->d_mm4 = nullptr;
     c->d_x = c->d_v = c->d_a = c->d_vh = c->d_tmp = c->d_ms = c->d_qs = nullptr;
@@ -95,6 +99,10 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
         const int l = std::atoi(g);
         if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32) c->nl.lanes = l;
     }
+    // spatial re-sort of the mixed path's state every 10,000 steps from 8,192 atoms on
+    // (NBX_SORT_EVERY: another interval for any size, 0 to never sort)
+    c->sort_every = n >= 8192 ? 10000 : 0;
+    if (const char* g = std::getenv("NBX_SORT_EVERY")) c->sort_every = std::max(0, std::atoi(g));
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     // set-up of nbx_rdf_noalloc: the device address of this GPU's histogram array
@@ -111,6 +119,18 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_qs, b1);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_red, b1 + 8192 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_scal, 16 * sizeof(double));
+    const size_t bi = sizeof(int) * (size_t)(n > 0 ? n : 1);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_perm, bi);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_iperm, bi);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_pmap, bi);
+    if (e == hipSuccess) {
+        c->h_perm.resize((size_t)n);
+        for (int64_t k = 0; k < n; ++k) c->h_perm[(size_t)k] = (int)k;
+        c->h_iperm = c->h_perm;
+        if (n > 0) e = hipMemcpy(c->d_perm, c->h_perm.data(), bi, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n > 0)
+            e = hipMemcpy(c->d_iperm, c->h_perm.data(), bi, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && n > 0)
         e = hipMemcpy(c->d_ms, masses, b1, hipMemcpyHostToDevice);
     if (e == hipSuccess && n > 0 && charges)
@@ -137,6 +157,7 @@ extern "C" int nbx_destroy(nbx_ctx* c) {
 static void invalidate(nbx_ctx* c) {
     c->accel_valid = false;
     c->nl.ready = false;
+    c->list_sorted = false;
     ++c->version;
 }
 
@@ -360,6 +381,7 @@ extern "C" int nbx_set_state(nbx_ctx* c, const double* u, const double* v) {
     c->accel_valid = false;
     c->nl.ready = false;
     c->steps_taken = 0;
+    if ((rc = mx_reset_order(c))) return rc;
     return reset_nosehoover(c);
 }
 

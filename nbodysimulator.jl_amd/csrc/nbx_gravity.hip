@@ -113,7 +113,7 @@ k_grav_finish(const double* __restrict__ part, int chunks, int n, double gscale,
             double v3[3];
             mx_kick_atom(i, ax, ay, az, A, kp, v3);
             if (kp.therm == NBX_THERM_ANDERSEN &&
-                nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
+                nbx_rng_uniform(kp.seed, mx_atom_id(kp, i), kp.step, NBX_RNG_ANDERSEN) < kp.prob)
                 mx_andersen_atom(i, kp.m0, kp, v3);
             io.v[i] = v3[0]; io.v[n + i] = v3[1]; io.v[2 * n + i] = v3[2];
             if (io.do_drift) tval = mx_drift_atom(i, n, v3, ax, ay, az, A, kp, io);

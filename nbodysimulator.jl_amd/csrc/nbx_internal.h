@@ -73,13 +73,15 @@ struct nbx_graph_key {
     int build_every = 1;      // steps between build opportunities
     int par_off = 0;          // buffer parity of the chunk step a replay starts after
     int em = 0;               // Euler-Maruyama steps instead of velocity Verlet
+    const void* ids = nullptr;  // slot -> user atom map of a sorted state (nullptr: none)
     bool operator==(const nbx_graph_key& o) const {
         return valid && o.valid && version == o.version && dt == o.dt && seed == o.seed &&
                nbr == o.nbr && nnb == o.nnb && cell_count == o.cell_count &&
                cell_atoms == o.cell_atoms && xref == o.xref && flags == o.flags && pos == o.pos &&
                part == o.part && cap == o.cap && ccap == o.ccap && ncell == o.ncell &&
                lanes == o.lanes && steps == o.steps && chunks == o.chunks && tpt == o.tpt &&
-               em == o.em && build_every == o.build_every && par_off == o.par_off;
+               em == o.em && build_every == o.build_every && par_off == o.par_off &&
+               ids == o.ids;
     }
 };
 
@@ -150,6 +152,21 @@ struct nbx_ctx {
     bool state_set = false;
     bool accel_valid = false; // d_a holds f(d_x, d_v) for the current state
     int64_t steps_taken = 0;
+    // Spatial order of the mixed path (Lennard-Jones-type Verlet-list runs): inside
+    // nbx_vv_run the per-atom state is kept in an order sorted by cell (slot s holds the
+    // user's atom d_perm[s]; d_iperm is the inverse), re-sorted at absolute steps that are
+    // multiples of sort_every, so that the neighbours of nearby atoms stay close in memory
+    // as the liquid mixes.  Outside a run the state is in the user's order; the Verlet
+    // list then belongs to the sorted order when list_sorted is set.  Thermostat random
+    // numbers are drawn for the user's atom index.
+    int sort_every = 0;       // 0: never sorted
+    int* d_perm = nullptr;    // slot -> user atom (identity when perm_identity)
+    int* d_iperm = nullptr;   // user atom -> slot
+    int* d_pmap = nullptr;    // scratch: a permutation being applied
+    std::vector<int> h_perm, h_iperm;
+    bool perm_identity = true;
+    bool list_sorted = false;
+    int64_t last_sort_step = -1;
     std::string err;
 };
 
@@ -250,3 +267,5 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
                    double tol, int check_every, double* report);
 int mx_count_pairs(nbx_ctx* c, int64_t* pairs);
 void mx_release(nbx_ctx* c);
+// the mixed path's sorted order back to the identity (a new state)
+int mx_reset_order(nbx_ctx* c);

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
+#include <vector>
 #include <cmath>
 #include <cstdlib>
 #include <string>
@@ -439,10 +441,17 @@ static int grow_lists(nbx_ctx* c, mx_plan& p, double& grow) {
     return prepare(c, p, grow);
 }
 
-static int ensure_ready(nbx_ctx* c, mx_plan& p) {
+// Lists and accelerations ready for the current state.  Outside a run (run = false) the
+// state is in the user's order, so a list built for a run's sorted order is rebuilt.
+static int ensure_ready(nbx_ctx* c, mx_plan& p, bool run = false) {
     int rc;
     double grow = 1.0;
+    if (!run && c->list_sorted) {
+        c->accel_valid = false;
+        c->list_sorted = false;
+    }
     if (!c->nl.ready || !c->accel_valid) {
+        c->list_sorted = false;   // the list made here is for the state's current order
         if ((rc = prepare(c, p, grow))) return rc;
         if ((rc = prologue(c, p))) return rc;
         int h[8];
@@ -459,11 +468,141 @@ static int ensure_ready(nbx_ctx* c, mx_plan& p) {
     return NBX_OK;
 }
 
+// ---- spatial order of the state during a run ----------------------------------------
+// Runs of mixed Lennard-Jones-type Verlet lists with per-step bins (no water molecules, no
+// Euler-Maruyama steps) keep the state sorted by cell when sort_every > 0: the list force
+// kernel's gathers then stay near each other in memory as the liquid mixes.
+static bool sort_enabled(const nbx_ctx* c, const mx_plan& p) {
+    return c->sort_every > 0 && c->n > 1 && p.kind == PLAN_NONBONDED && p.nb.mode_list &&
+           !p.nb.water && c->nl.ncell >= 3 && c->nl.bin_mode == NBX_BINS_STEP && !c->em;
+}
+
+int mx_reset_order(nbx_ctx* c) {
+    const int64_t n = c->n;
+    for (int64_t k = 0; k < n; ++k) c->h_perm[(size_t)k] = (int)k;
+    c->h_iperm = c->h_perm;
+    if (n > 0) {
+        NBX_HIP(c, hipMemcpyAsync(c->d_perm, c->h_perm.data(), sizeof(int) * n,
+                                  hipMemcpyHostToDevice, c->stream));
+        NBX_HIP(c, hipMemcpyAsync(c->d_iperm, c->h_perm.data(), sizeof(int) * n,
+                                  hipMemcpyHostToDevice, c->stream));
+        NBX_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->perm_identity = true;
+    c->list_sorted = false;
+    c->last_sort_step = -1;
+    return NBX_OK;
+}
+
+// the per-atom state reordered in place: new[s] = old[map[s]] (positions, velocities,
+// accelerations, half-kicked velocities, list reference positions, masses, charges)
+static int permute_state(nbx_ctx* c, const int* d_map) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    double* a3[5] = {c->d_x, c->d_v, c->d_a, c->d_vh, c->nl.d_xref};
+    for (double* a : a3) {
+        if (!a) continue;
+        mx_launch_gather(s, a, c->d_tmp, d_map, n, 3);
+        NBX_HIP(c, hipMemcpyAsync(a, c->d_tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+    }
+    double* a1[2] = {c->d_ms, c->has_q ? c->d_qs : nullptr};
+    for (double* a : a1) {
+        if (!a) continue;
+        mx_launch_gather(s, a, c->d_tmp, d_map, n, 1);
+        NBX_HIP(c, hipMemcpyAsync(a, c->d_tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    return NBX_OK;
+}
+
+// a Verlet list for the current positions (fresh bins, gated build with the request set,
+// grown lists on overflow); the accelerations stay as they are
+static int rebuild_list(nbx_ctx* c, mx_plan& p) {
+    int rc;
+    double grow = 1.0;
+    for (int attempt = 0;; ++attempt) {
+        refresh_positions(c, p, true);
+        launch_build(c, p, 0);
+        NBX_HIP(c, hipGetLastError());
+        int h[8];
+        if ((rc = read_flags(c, h))) return rc;
+        if (!h[1]) return NBX_OK;
+        if (attempt > 8) return nbx_fail(c, NBX_ERR_CAPACITY, "list overflow persists");
+        if ((rc = grow_lists(c, p, grow))) return rc;
+    }
+}
+
+// a new sorted order from the current positions: slots by cell of the list grid
+// (z-major), the atoms of a cell by user index; the state is reordered and the list
+// rebuilt.  The state must be in the current sorted order.
+static int resort(nbx_ctx* c, mx_plan& p) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    std::vector<double> x(3 * (size_t)n);
+    NBX_HIP(c, hipMemcpyAsync(x.data(), c->d_x, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+    NBX_HIP(c, hipStreamSynchronize(s));
+    const double L = c->bc.b[0];
+    const int g = c->nl.ncell;
+    std::vector<std::pair<int64_t, int>> key((size_t)n);
+    for (int64_t sl = 0; sl < n; ++sl) {
+        int64_t cell = 0;
+        for (int k = 2; k >= 0; --k) {
+            const double xk = x[(size_t)(k * n + sl)];
+            const double w = xk - L * std::floor(xk / L);
+            int ci = (int)(w / L * g);
+            ci = ci < 0 ? 0 : (ci >= g ? g - 1 : ci);
+            cell = cell * g + ci;
+        }
+        key[(size_t)sl] = {cell * n + c->h_perm[(size_t)sl], (int)sl};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> map((size_t)n), perm((size_t)n), iperm((size_t)n);
+    for (int64_t t = 0; t < n; ++t) {
+        map[(size_t)t] = key[(size_t)t].second;                 // new slot t <- old slot
+        perm[(size_t)t] = c->h_perm[(size_t)key[(size_t)t].second];
+    }
+    for (int64_t t = 0; t < n; ++t) iperm[(size_t)perm[(size_t)t]] = (int)t;
+    int rc;
+    NBX_HIP(c, hipMemcpyAsync(c->d_pmap, map.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    if ((rc = permute_state(c, c->d_pmap))) return rc;
+    NBX_HIP(c, hipMemcpyAsync(c->d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    NBX_HIP(c, hipMemcpyAsync(c->d_iperm, iperm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    NBX_HIP(c, hipStreamSynchronize(s));
+    c->h_perm.swap(perm);
+    c->h_iperm.swap(iperm);
+    c->perm_identity = false;
+    return rebuild_list(c, p);
+}
+
 int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every, double* uf,
           double* vf, int64_t* nframes) {
     mx_plan p;
     int rc;
-    if ((rc = ensure_ready(c, p))) return rc;
+    if ((rc = ensure_ready(c, p, true))) return rc;
+    // the run keeps the state in the sorted order and returns it in the user's order
+    const bool sorting = sort_enabled(c, p);
+    struct order_guard {
+        nbx_ctx* c = nullptr;
+        ~order_guard() {
+            if (!c || c->perm_identity) return;
+            (void)permute_state(c, c->d_iperm);
+            (void)hipStreamSynchronize(c->stream);
+            c->list_sorted = true;
+        }
+    } og;
+    if (sorting) {
+        if (!c->perm_identity) {
+            if ((rc = permute_state(c, c->d_perm))) return rc;
+            og.c = c;
+            if (!c->list_sorted && (rc = rebuild_list(c, p))) return rc;
+        } else {
+            og.c = c;
+        }
+    } else if (c->list_sorted) {
+        // a list of another run's sorted order (the plan changed since)
+        c->list_sorted = false;
+        c->accel_valid = false;
+        if ((rc = ensure_ready(c, p, true))) return rc;
+    }
     double grow = 1.0;
     hipStream_t s = c->stream;
     const int64_t n = c->n;
@@ -494,6 +633,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
             mx_kick_params kp = kick_params(c, 1, dt, seed);
             kp.step_base = c->d_step;
             kp.step_off = k - k0;
+            kp.ids = sorting ? c->d_perm : nullptr;
             if ((base + k - 1) % M == 0) launch_build(c, p, par);
             launch_step(c, p, kp, par, 1, true, ((base + k) % M == 0 || k == last) ? 1 : 0);
         }
@@ -523,6 +663,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         key.steps = gsteps;
         key.build_every = build_every(c);
         key.par_off = par_off;
+        key.ids = sorting ? c->d_perm : nullptr;
         if (c->gexec && key == c->gkey) { *out = c->gexec; return NBX_OK; }
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         hipGraph_t g = nullptr;
@@ -545,6 +686,16 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
     while (done < nsteps) {
         int64_t chunk = std::min<int64_t>(chunk_max, nsteps - done);
         if (save_every > 0) chunk = std::min(chunk, save_every - (done % save_every));
+        if (sorting) {
+            // re-sorted at absolute steps that are multiples of sort_every (the chunks end
+            // there), so that split runs keep the order of an unsplit one
+            const int64_t S = c->sort_every, t0 = c->steps_taken;
+            if (t0 % S == 0 && c->last_sort_step != t0) {
+                if ((rc = resort(c, p))) return rc;
+                c->last_sort_step = t0;
+            }
+            chunk = std::min(chunk, S - t0 % S);
+        }
         // device-side backup of the state for an overflow replay
         const double* src[4] = {c->d_x, c->d_v, c->d_a, c->d_vh};
         for (int k = 0; k < 4; ++k)
@@ -568,6 +719,7 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
                 mx_kick_params kp = kick_params(c, 1, dt, seed);
                 kp.step_base = c->d_step;
                 kp.step_off = 0;
+                kp.ids = sorting ? c->d_perm : nullptr;
                 const int first_opp = (step0 % build_every(c)) == 0 ? 1 : 0;
                 const mx_io io0 = make_io(c, p, 0, 1, dt, first_opp);
                 mx_launch_drift(s, c, kp, io0);
@@ -626,11 +778,20 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         done += chunk;
         c->steps_taken = step0 + chunk;
         if (save_every > 0 && uf && done % save_every == 0) {
-            ref64_soa_to_aos(s, c->d_x, c->d_tmp, n);
+            // frames in the user's order (the backup buffer is free between chunks)
+            const double* fx = c->d_x;
+            const double* fv = c->d_v;
+            if (sorting && !c->perm_identity) {
+                mx_launch_gather(s, c->d_x, bk.p, c->d_iperm, n, 3);
+                mx_launch_gather(s, c->d_v, bk.p + n3, c->d_iperm, n, 3);
+                fx = bk.p;
+                fv = bk.p + n3;
+            }
+            ref64_soa_to_aos(s, fx, c->d_tmp, n);
             NBX_HIP(c, hipMemcpyAsync(uf + *nframes * n3, c->d_tmp, sizeof(double) * n3,
                                       hipMemcpyDeviceToHost, s));
             NBX_HIP(c, hipStreamSynchronize(s));
-            ref64_soa_to_aos(s, c->d_v, c->d_tmp, n);
+            ref64_soa_to_aos(s, fv, c->d_tmp, n);
             NBX_HIP(c, hipMemcpyAsync(vf + *nframes * n3, c->d_tmp, sizeof(double) * n3,
                                       hipMemcpyDeviceToHost, s));
             NBX_HIP(c, hipStreamSynchronize(s));
@@ -697,7 +858,8 @@ int mx_count_pairs(nbx_ctx* c, int64_t* pairs) {
         return nbx_fail(c, NBX_ERR_UNSUPPORTED, "pair count needs a cubic box with finite cutoffs");
     const int prec = c->prec;
     int rc;
-    if (!c->nl.ready || !c->accel_valid || prec != NBX_PREC_MIXED) {
+    // a list left by a sorted run is for the cell order, the state is in the user's
+    if (!c->nl.ready || !c->accel_valid || c->list_sorted || prec != NBX_PREC_MIXED) {
         c->prec = NBX_PREC_MIXED;
         rc = ensure_ready(c, p);
         c->prec = prec;

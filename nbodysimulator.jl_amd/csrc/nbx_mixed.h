@@ -46,7 +46,14 @@ struct mx_kick_params {
     // velocity Verlet: x' = x + dt v, v' = v + dt (a - gamma v) + sigma sqrt(dt) xi
     int em;
     double em_gamma, em_sigma, em_sqrt_dt;
+    // slot -> user atom index when the state is kept in a sorted order (random numbers are
+    // drawn for the user's atom), nullptr for the identity
+    const int* ids;
 };
+
+__device__ __forceinline__ int64_t mx_atom_id(const mx_kick_params& kp, int64_t i) {
+    return kp.ids != nullptr ? (int64_t)kp.ids[i] : i;
+}
 
 // Nose-Hoover update at the end of a temperature reduction (src/thermostats.jl:116-128):
 // zeta += dt xi (advance), then xi = (1/tau)^2 (T/T0 - (ndf+1)/ndf)
@@ -138,7 +145,7 @@ __device__ __forceinline__ void mx_kick_atom(int64_t i, double ax, double ay, do
     if (kp.therm == NBX_THERM_LANGEVIN) {
         const double sig = kp.c2 * sqrt(kp.kb * kp.T / A.m);
         double g[3];
-        nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_LANGEVIN_B, g);
+        nbx_rng_normals3(kp.seed, mx_atom_id(kp, i), kp.step, NBX_RNG_LANGEVIN_B, g);
         for (int k = 0; k < 3; ++k) v3[k] = kp.c1 * v3[k] + sig * g[k];
     }
 }
@@ -148,7 +155,7 @@ __device__ __forceinline__ void mx_andersen_atom(int64_t i, double m, const mx_k
                                                  double* v3) {
     const double vdev = sqrt(kp.kb * kp.T / m);
     double g[3];
-    nbx_rng_normals3(kp.seed, i, kp.step, NBX_RNG_ANDERSEN, g);
+    nbx_rng_normals3(kp.seed, mx_atom_id(kp, i), kp.step, NBX_RNG_ANDERSEN, g);
     for (int k = 0; k < 3; ++k) v3[k] = vdev * g[k];
 }
 
@@ -208,7 +215,7 @@ __device__ __forceinline__ double mx_em_atom(int64_t i, int64_t n, double vx, do
     io.x_out[n + i] = py;
     io.x_out[2 * n + i] = pz;
     double g[3];
-    nbx_rng_normals3(kp.seed, i, kp.step + 1, NBX_RNG_EM, g);
+    nbx_rng_normals3(kp.seed, mx_atom_id(kp, i), kp.step + 1, NBX_RNG_EM, g);
     io.vh[i] = vx + dt * (ax - kp.em_gamma * vx) + kp.em_sigma * (kp.em_sqrt_dt * g[0]);
     io.vh[n + i] = vy + dt * (ay - kp.em_gamma * vy) + kp.em_sigma * (kp.em_sqrt_dt * g[1]);
     io.vh[2 * n + i] = vz + dt * (az - kp.em_gamma * vz) + kp.em_sigma * (kp.em_sqrt_dt * g[2]);
@@ -229,7 +236,7 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
     if (kp.therm == NBX_THERM_LANGEVIN) {
         const double sig = kp.c2 * sqrt(kp.kb * kp.T / mi);
         double g[3];
-        nbx_rng_normals3(kp.seed, i, kp.step + 1, NBX_RNG_LANGEVIN_A, g);
+        nbx_rng_normals3(kp.seed, mx_atom_id(kp, i), kp.step + 1, NBX_RNG_LANGEVIN_A, g);
         vx = kp.c1 * vx + sig * g[0];
         vy = kp.c1 * vy + sig * g[1];
         vz = kp.c1 * vz + sig * g[2];
@@ -308,6 +315,9 @@ void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
 // with io.need_T: the temperature of vh_{k+1} from the block partials of the step kernel
 // just launched (and the Nose-Hoover update)
 void mx_launch_temperature_finish(hipStream_t s, const mx_io& io);
+// dst[k * n + s] = src[k * n + map[s]] for k < comps (map: a permutation of [0, n))
+void mx_launch_gather(hipStream_t s, const double* src, double* dst, const int* map,
+                      int64_t n, int comps);
 // Verlet list (gated by req) from bins B; its last block clears req and, with
 // clear_bins, the counts of B (NBX_BINS_BUILD: B is the base of the three rotating sets,
 // which the build advances)

@@ -97,7 +97,7 @@ __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* 
     double v3[3];
     mx_kick_atom(i, ax, ay, az, A, kp, v3);
     if (kp.therm == NBX_THERM_ANDERSEN &&
-        nbx_rng_uniform(kp.seed, i, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
+        nbx_rng_uniform(kp.seed, mx_atom_id(kp, i), kp.step, NBX_RNG_ANDERSEN) < kp.prob)
         mx_andersen_atom(i, kp.m0, kp, v3);
     io.v[i] = v3[0]; io.v[n + i] = v3[1]; io.v[2 * n + i] = v3[2];
     if (!io.do_drift) return 0.0;
@@ -427,6 +427,24 @@ __global__ void __launch_bounds__(256) k_temperature_finish(mx_io io) {
 
 void mx_launch_temperature_finish(hipStream_t s, const mx_io& io) {
     hipLaunchKernelGGL(k_temperature_finish, dim3(1), dim3(256), 0, s, io);
+}
+
+// one component-major (SoA) array reordered by a permutation
+__global__ void __launch_bounds__(256) k_gather(const double* __restrict__ src,
+                                                double* __restrict__ dst,
+                                                const int* __restrict__ map, int64_t n,
+                                                int comps) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t j = map[s];
+    for (int k = 0; k < comps; ++k) dst[k * n + s] = src[k * n + j];
+}
+
+void mx_launch_gather(hipStream_t s, const double* src, double* dst, const int* map,
+                      int64_t n, int comps) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, src, dst, map, n,
+                       comps);
 }
 
 // ---------------------------------------------------------------------------
@@ -909,7 +927,7 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
             double v3[3];
             mx_kick_atom(i, acc[0], acc[1], acc[2], A, kp, v3);
             if (kp.therm == NBX_THERM_ANDERSEN &&
-                nbx_rng_uniform(kp.seed, o, kp.step, NBX_RNG_ANDERSEN) < kp.prob)
+                nbx_rng_uniform(kp.seed, mx_atom_id(kp, o), kp.step, NBX_RNG_ANDERSEN) < kp.prob)
                 mx_andersen_atom(i, mi, kp, v3);
             io.v[i] = v3[0];
             io.v[n + i] = v3[1];
