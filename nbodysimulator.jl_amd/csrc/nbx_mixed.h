@@ -87,6 +87,8 @@ struct mx_io {
     int* req_out;         // rebuild request for step k+1
     int* flags;           // [1] overflow
     double L;             // cubic box side (wrapped copy)
+    double invL;          // 1 / L
+    int fixq;             // also write the fixed-point copy (mx_posq) of the positions
     // Berendsen temperature of vh_{k+1}
     int need_T;
     double* red_part;     // one partial per block
@@ -180,12 +182,36 @@ __device__ __forceinline__ double mx_wrap(double p, double L) {
     return w >= L ? w - L : w;
 }
 
+// Fixed-point copy of the wrapped positions (Lennard-Jones-only list plans): the box
+// fraction of each coordinate as an unsigned 32-bit integer, stored after the n double4 of
+// a position buffer.  The difference of two such words, read as a signed integer, is the
+// minimum-image displacement in units of L / 2^32 (exact: the wrap-around of the integer
+// subtraction is the periodic image), and a gather moves 16 bytes instead of 32.
+__device__ __forceinline__ uint4* mx_posq(void* pos, int64_t n) {
+    return reinterpret_cast<uint4*>(static_cast<double4*>(pos) + n);
+}
+__device__ __forceinline__ const uint4* mx_posq(const void* pos, int64_t n) {
+    return reinterpret_cast<const uint4*>(static_cast<const double4*>(pos) + n);
+}
+// box fraction x / L - floor(x / L) in [0, 1) (the subtraction is exact) times 2^32
+__device__ __forceinline__ uint32_t mx_frac32(double x, double invL) {
+    double t = x * invL;
+    t -= floor(t);
+    return (uint32_t)(uint64_t)(t * 4294967296.0);
+}
+__device__ __forceinline__ uint4 mx_fixed_position(double px, double py, double pz,
+                                                   double invL) {
+    return make_uint4(mx_frac32(px, invL), mx_frac32(py, invL), mx_frac32(pz, invL), 0u);
+}
+
 // wrapped copy of x_{k+1} and the skin test against the positions of the last build
-__device__ __forceinline__ void mx_store_position(int64_t i, double px, double py, double pz,
-                                                  const mx_atom& A, const mx_io& io) {
+__device__ __forceinline__ void mx_store_position(int64_t i, int64_t n, double px, double py,
+                                                  double pz, const mx_atom& A,
+                                                  const mx_io& io) {
     if (io.pos_kind == 1) {
         const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
         ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, A.q);
+        if (io.fixq) mx_posq(io.pos_out, n)[i] = mx_fixed_position(px, py, pz, io.invL);
     } else if (io.pos_kind == 2) {
         ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz,
                                                (float)(A.m * io.wscale));
@@ -219,7 +245,7 @@ __device__ __forceinline__ double mx_em_atom(int64_t i, int64_t n, double vx, do
     io.vh[i] = vx + dt * (ax - kp.em_gamma * vx) + kp.em_sigma * (kp.em_sqrt_dt * g[0]);
     io.vh[n + i] = vy + dt * (ay - kp.em_gamma * vy) + kp.em_sigma * (kp.em_sqrt_dt * g[1]);
     io.vh[2 * n + i] = vz + dt * (az - kp.em_gamma * vz) + kp.em_sigma * (kp.em_sqrt_dt * g[2]);
-    mx_store_position(i, px, py, pz, A, io);
+    mx_store_position(i, n, px, py, pz, A, io);
     return 0.0;
 }
 
@@ -252,7 +278,7 @@ __device__ __forceinline__ double mx_drift_atom(int64_t i, int64_t n, const doub
     io.vh[i] = hx;
     io.vh[n + i] = hy;
     io.vh[2 * n + i] = hz;
-    mx_store_position(i, px, py, pz, A, io);
+    mx_store_position(i, n, px, py, pz, A, io);
     return mi * (hx * hx + hy * hy + hz * hz);
 }
 
@@ -326,7 +352,8 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
                      int clear_bins);
 // wrapped copy of x into pos, binned into bins; sets *req
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
-                         int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req);
+                         int pos_kind, int fixq, double wscale, int* bin_count, int* bin_atoms,
+                         int* req);
 void mx_launch_temperature(hipStream_t s, const nbx_ctx* c, const double* v, int64_t Nc,
                            double* d_T);
 void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& L,

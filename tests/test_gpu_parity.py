@@ -296,6 +296,22 @@ def test_mixed_gravity_si_masses_stay_finite():
     assert relerr(ag, ref) <= MIXED_TOL
 
 
+def test_mixed_lj_periodic_images_of_the_fixed_point_positions(c1):
+    """The Lennard-Jones list kernel takes displacements from 32-bit box fractions (the
+    integer difference is the minimum image): atoms moved by whole boxes, up to 3 L either
+    way, give the same forces to the rounding of their fractions, and both match the oracle.
+    A dropped wrap or a sign error in the fraction would move a shifted atom's image."""
+    print("[case] mixed LJ N=864, atoms shifted by whole boxes", flush=True)
+    rng = np.random.default_rng(17)
+    shifted = c1.u + c1.L * rng.integers(-3, 4, size=c1.u.shape)
+    ctx, orc = lj_pair(864, c1.L, prec=_lib.PREC_MIXED)
+    zero = np.zeros_like(c1.u)
+    a0 = _mixed_accel(ctx, c1.u, zero)
+    a1 = _mixed_accel(ctx, shifted, zero)
+    assert relerr(a1, a0) <= 1e-6
+    assert relerr(a1, orc.rhs(c1.u)) <= MIXED_TOL
+
+
 def test_mixed_all_pairs_mode_matches(c1):
     # periodic lo/hi wrapping is not minimum image: the mixed path uses all pairs
     ctx, orc = lj_pair(864, c1.L, prec=_lib.PREC_MIXED, bc_kind=_lib.BC_PERIODIC)
