@@ -582,10 +582,10 @@ __global__ void __launch_bounds__(256) k_lr_drift(const double* __restrict__ par
         io.x_out[(size_t)k * n + i] = p3[k];
         io.vh[(size_t)k * n + i] = h3[k];
     }
+    const double qi = io.qs != nullptr ? io.qs[i] : 0.0;
     ((double4*)io.pos_out)[i] =
-        make_double4(mx_wrap(p3[0], io.L), mx_wrap(p3[1], io.L), mx_wrap(p3[2], io.L),
-                     io.qs != nullptr ? io.qs[i] : 0.0);
-    if (io.fixq) mx_posq(io.pos_out, n)[i] = mx_fixed_position(p3[0], p3[1], p3[2], io.invL);
+        make_double4(mx_wrap(p3[0], io.L), mx_wrap(p3[1], io.L), mx_wrap(p3[2], io.L), qi);
+    mx_posq(io.pos_out, n)[i] = mx_fixed_position(p3[0], p3[1], p3[2], io.invL, qi);
     const double ex = p3[0] - io.xref[i], ey = p3[1] - io.xref[n + i], ez = p3[2] - io.xref[2 * n + i];
     const double e2 = ex * ex + ey * ey + ez * ez;
     if (e2 > io.half_skin2) io.req_out[0] = 1;

@@ -202,13 +202,6 @@ int pos_kind(const mx_plan& p) {
     return p.nb.mode_list ? 1 : 0;
 }
 
-// Lennard-Jones-only list plans: the position buffers also hold the fixed-point copy that
-// their list force kernel gathers (the kernel takes that path for exactly these plans)
-bool fixq_plan(const mx_plan& p) {
-    return p.kind == PLAN_NONBONDED && p.nb.mode_list && p.nb.has_lj && p.nb.el_kind == 0 &&
-           !p.nb.water;
-}
-
 mx_kick_params kick_params(const nbx_ctx* c, int do_kick, double dt, uint64_t seed) {
     mx_kick_params kp{};
     kp.do_kick = do_kick;
@@ -293,7 +286,6 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
         io.req_out = req_flag(c);
         io.L = c->bc.b[0];
         io.invL = 1.0 / io.L;
-        io.fixq = fixq_plan(p) ? 1 : 0;
     } else {
         io.next_opp = 1;
     }
@@ -415,8 +407,7 @@ static void refresh_positions(nbx_ctx* c, const mx_plan& p, bool bins) {
         mx_launch_fill_int(c->stream, c->nl.d_cell_count, (int64_t)(3 * ncells), 0);
         mx_launch_fill_int(c->stream, c->nl.d_flags + NBX_FLAG_BINSET, 2, 0);
     }
-    mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, fixq_plan(p) ? 1 : 0,
-                        mass_wscale(c, p),
+    mx_launch_positions(c->stream, c, c->d_x, c->d_pos, pk, mass_wscale(c, p),
                         list ? bin_count(c) : nullptr,
                         list ? bin_atoms(c) : nullptr,
                         (pk == 1 && bins) ? req_flag(c) : nullptr);

@@ -68,7 +68,7 @@ struct mx_io {
     const double* x_in;   // x_k (SoA, unwrapped)
     double* x_out;        // x_{k+1}
     void* pos_out;        // wrapped copy of x_{k+1}: double4 (+q) or float4 (+m wscale)
-    int pos_kind;         // 0 none, 1 double4 wrapped, 2 float4
+    int pos_kind;         // 0 none, 1 double4 wrapped (+ fixed-point copy), 2 float4
     double wscale;        // pos_kind 2: factor on the mass in .w (gravity: 1 / m_max)
     double* a;            // a_k out
     double* v;            // v_k out
@@ -87,8 +87,7 @@ struct mx_io {
     int* req_out;         // rebuild request for step k+1
     int* flags;           // [1] overflow
     double L;             // cubic box side (wrapped copy)
-    double invL;          // 1 / L
-    int fixq;             // also write the fixed-point copy (mx_posq) of the positions
+    double invL;          // 1 / L (the fixed-point copy, mx_posq)
     // Berendsen temperature of vh_{k+1}
     int need_T;
     double* red_part;     // one partial per block
@@ -182,11 +181,11 @@ __device__ __forceinline__ double mx_wrap(double p, double L) {
     return w >= L ? w - L : w;
 }
 
-// Fixed-point copy of the wrapped positions (Lennard-Jones-only list plans): the box
-// fraction of each coordinate as an unsigned 32-bit integer, stored after the n double4 of
-// a position buffer.  The difference of two such words, read as a signed integer, is the
-// minimum-image displacement in units of L / 2^32 (exact: the wrap-around of the integer
-// subtraction is the periodic image), and a gather moves 16 bytes instead of 32.
+// Fixed-point copy of the wrapped positions (list plans): the box fraction of each
+// coordinate as an unsigned 32-bit integer and the charge as FP32 bits, stored after the n
+// double4 of a position buffer.  The difference of two such words, read as a signed
+// integer, is the minimum-image displacement in units of L / 2^32 (exact: the wrap-around
+// of the integer subtraction is the periodic image), and a gather moves 16 bytes, not 32.
 __device__ __forceinline__ uint4* mx_posq(void* pos, int64_t n) {
     return reinterpret_cast<uint4*>(static_cast<double4*>(pos) + n);
 }
@@ -200,8 +199,9 @@ __device__ __forceinline__ uint32_t mx_frac32(double x, double invL) {
     return (uint32_t)(uint64_t)(t * 4294967296.0);
 }
 __device__ __forceinline__ uint4 mx_fixed_position(double px, double py, double pz,
-                                                   double invL) {
-    return make_uint4(mx_frac32(px, invL), mx_frac32(py, invL), mx_frac32(pz, invL), 0u);
+                                                   double invL, double q) {
+    return make_uint4(mx_frac32(px, invL), mx_frac32(py, invL), mx_frac32(pz, invL),
+                      __float_as_uint((float)q));
 }
 
 // wrapped copy of x_{k+1} and the skin test against the positions of the last build
@@ -211,7 +211,7 @@ __device__ __forceinline__ void mx_store_position(int64_t i, int64_t n, double p
     if (io.pos_kind == 1) {
         const double wx = mx_wrap(px, io.L), wy = mx_wrap(py, io.L), wz = mx_wrap(pz, io.L);
         ((double4*)io.pos_out)[i] = make_double4(wx, wy, wz, A.q);
-        if (io.fixq) mx_posq(io.pos_out, n)[i] = mx_fixed_position(px, py, pz, io.invL);
+        mx_posq(io.pos_out, n)[i] = mx_fixed_position(px, py, pz, io.invL, A.q);
     } else if (io.pos_kind == 2) {
         ((float4*)io.pos_out)[i] = make_float4((float)px, (float)py, (float)pz,
                                                (float)(A.m * io.wscale));
@@ -352,8 +352,7 @@ void mx_launch_build(hipStream_t s, const nbx_ctx* c, int water, const void* pos
                      int clear_bins);
 // wrapped copy of x into pos, binned into bins; sets *req
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
-                         int pos_kind, int fixq, double wscale, int* bin_count, int* bin_atoms,
-                         int* req);
+                         int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req);
 void mx_launch_temperature(hipStream_t s, const nbx_ctx* c, const double* v, int64_t Nc,
                            double* d_T);
 void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& L,

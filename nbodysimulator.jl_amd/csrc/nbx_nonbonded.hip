@@ -7,10 +7,10 @@
 //                    every atom while writing x_{k+1})
 //   P[2]           : double4 per atom = x wrapped into [0, L)^3 and the charge; pair
 //                    displacements are taken in FP64 from these, minimum-imaged, then
-//                    cast to FP32 for the pair law.  Lennard-Jones-only plans also keep,
-//                    after the n double4, a fixed-point copy (mx_posq: the box fraction of
-//                    each coordinate in 32 bits) that their list force kernel gathers
-//                    instead: the integer difference is the minimum image
+//                    cast to FP32 for the pair law (all-pairs kernels).  After the n
+//                    double4 follows a fixed-point copy (mx_posq: the box fraction of each
+//                    coordinate in 32 bits, the charge in FP32) that the list force kernel
+//                    gathers: the integer difference is the minimum image
 //   nbr            : Verlet list, interleaved for G lanes per atom: slot k of atom i is
 //                    at (k / G) * (n * G) + i * G + (k % G), so lane l of atom i reads
 //                    consecutive words with the other lanes of the wavefront
@@ -31,9 +31,6 @@
 
 #define NB_THREADS 256
 #define NB_UNROLL 4
-#ifndef NB_UNROLL_FIX
-#define NB_UNROLL_FIX 4
-#endif
 
 // ---------------------------------------------------------------------------
 // Pair law
@@ -118,7 +115,6 @@ template <int G, bool LJ, int EL, bool WATER>
 __global__ void __launch_bounds__(NB_THREADS)
 k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
           const int* __restrict__ nnb, int n, mx_pair_params pp, mx_kick_params kp, mx_io io) {
-    constexpr bool FIX = LJ && EL == 0 && !WATER;
     const int gid = mx_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     mx_resolve_step(kp);
     const int i = gid / G;
@@ -145,45 +141,21 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
         // NB_UNROLL slots per lane and pass: the position gathers of a pass are issued
         // together, and the slots of the next pass are loaded while this pass computes,
         // so a lane waits for about one gather per pass rather than two dependent loads
-        // per pair (slots past the count read the atom itself and are skipped)
-        if constexpr (FIX) {
-            // Lennard-Jones only: 16-byte fixed-point positions, integer minimum image
-            const uint4* q = mx_posq(pos, n);
-            const uint4 qi = q[i];
-            const float sc = (float)(pp.L * 0x1p-32);
-            int jn[NB_UNROLL_FIX];
-#pragma unroll
-            for (int u = 0; u < NB_UNROLL_FIX; ++u) jn[u] = lane + u * G < cnt ? p[u * stride] : i;
-            for (int k = lane; k < cnt; k += NB_UNROLL_FIX * G) {
-                uint4 pq[NB_UNROLL_FIX];
-#pragma unroll
-                for (int u = 0; u < NB_UNROLL_FIX; ++u) pq[u] = q[jn[u]];
-                p += NB_UNROLL_FIX * stride;
-                const int kn = k + NB_UNROLL_FIX * G;
-#pragma unroll
-                for (int u = 0; u < NB_UNROLL_FIX; ++u)
-                    jn[u] = kn + u * G < cnt ? p[u * stride] : i;
-#pragma unroll
-                for (int u = 0; u < NB_UNROLL_FIX; ++u) {
-                    if (k + u * G >= cnt) break;
-                    const float rx = (float)(int)(qi.x - pq[u].x) * sc;
-                    const float ry = (float)(int)(qi.y - pq[u].y) * sc;
-                    const float rz = (float)(int)(qi.z - pq[u].z) * sc;
-                    nb_pair<LJ, EL>(rx, ry, rz, 0.0f, true, pp, acc);
-                }
-            }
-        } else {
-        const double4 pi = pos[i];
+        // per pair (slots past the count read the atom itself and are skipped).  The
+        // gathers read the 16-byte fixed-point copy: integer minimum image, FP32 charge
+        const uint4* qpos = mx_posq(pos, n);
+        const uint4 qi = qpos[i];
+        const float sc = (float)(pp.L * 0x1p-32);
         int jn[NB_UNROLL];
 #pragma unroll
         for (int u = 0; u < NB_UNROLL; ++u) jn[u] = lane + u * G < cnt ? p[u * stride] : i;
         for (int k = lane; k < cnt; k += NB_UNROLL * G) {
             int jj[NB_UNROLL];
-            double4 pq[NB_UNROLL];
+            uint4 pq[NB_UNROLL];
 #pragma unroll
             for (int u = 0; u < NB_UNROLL; ++u) jj[u] = jn[u];
 #pragma unroll
-            for (int u = 0; u < NB_UNROLL; ++u) pq[u] = pos[jj[u]];
+            for (int u = 0; u < NB_UNROLL; ++u) pq[u] = qpos[jj[u]];
             p += NB_UNROLL * stride;
             const int kn = k + NB_UNROLL * G;
 #pragma unroll
@@ -191,15 +163,12 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
 #pragma unroll
             for (int u = 0; u < NB_UNROLL; ++u) {
                 if (k + u * G >= cnt) break;
-                const double4 pj = pq[u];
-                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                dx -= pp.L * rint(dx * pp.invL);
-                dy -= pp.L * rint(dy * pp.invL);
-                dz -= pp.L * rint(dz * pp.invL);
+                const float rx = (float)(int)(qi.x - pq[u].x) * sc;
+                const float ry = (float)(int)(qi.y - pq[u].y) * sc;
+                const float rz = (float)(int)(qi.z - pq[u].z) * sc;
                 const bool ljp = io_ && (!WATER || (jj[u] % 3) == 0);
-                nb_pair<LJ, EL>((float)dx, (float)dy, (float)dz, (float)pj.w, ljp, pp, acc);
+                nb_pair<LJ, EL>(rx, ry, rz, __uint_as_float(pq[u].w), ljp, pp, acc);
             }
-        }
         }
         double F[6];
 #pragma unroll
@@ -363,9 +332,8 @@ void mx_launch_fill_int(hipStream_t s, int* p, int64_t count, int value) {
 }
 
 // wrapped copy of x (after set_state), binned; requests a list build
-__global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, int fixq,
-                               void* pos, double L, double wscale,
-                               const double* __restrict__ ms,
+__global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, void* pos,
+                               double L, double wscale, const double* __restrict__ ms,
                                const double* __restrict__ qs, int* __restrict__ bin_count,
                                int* __restrict__ bin_atoms, int ccap, int nc, double inv_cs,
                                int* flags, int* req) {
@@ -375,8 +343,9 @@ __global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, in
     const double px = x[i], py = x[n + i], pz = x[2 * n + i];
     if (kind == 1) {
         const double wx = mx_wrap(px, L), wy = mx_wrap(py, L), wz = mx_wrap(pz, L);
-        ((double4*)pos)[i] = make_double4(wx, wy, wz, qs != nullptr ? qs[i] : 0.0);
-        if (fixq) mx_posq(pos, n)[i] = mx_fixed_position(px, py, pz, 1.0 / L);
+        const double q = qs != nullptr ? qs[i] : 0.0;
+        ((double4*)pos)[i] = make_double4(wx, wy, wz, q);
+        mx_posq(pos, n)[i] = mx_fixed_position(px, py, pz, 1.0 / L, q);
         if (bin_count != nullptr) {
             const int cid = (mx_cell_coord(wz, inv_cs, nc) * nc + mx_cell_coord(wy, inv_cs, nc)) *
                                 nc + mx_cell_coord(wx, inv_cs, nc);
@@ -390,13 +359,11 @@ __global__ void k_mx_positions(const double* __restrict__ x, int n, int kind, in
 }
 
 void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void* pos,
-                         int pos_kind, int fixq, double wscale, int* bin_count, int* bin_atoms,
-                         int* req) {
+                         int pos_kind, double wscale, int* bin_count, int* bin_atoms, int* req) {
     if (c->n <= 0) return;
     const int nc = c->nl.ncell;
     hipLaunchKernelGGL(k_mx_positions, dim3(nbx_blocks(c->n, 256)), dim3(256), 0, s, x,
-                       (int)c->n, pos_kind, pos_kind == 1 ? fixq : 0, pos, c->bc.b[0], wscale,
-                       c->d_ms,
+                       (int)c->n, pos_kind, pos, c->bc.b[0], wscale, c->d_ms,
                        c->has_q ? c->d_qs : nullptr, nc >= 3 ? bin_count : nullptr, bin_atoms,
                        c->nl.ccap, nc, nc > 0 ? (double)nc / c->bc.b[0] : 0.0, c->nl.d_flags, req);
 }
