@@ -1345,7 +1345,21 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     const std::string key = tune_key(c, p, grav, dt, trial);
     {
         tune_result r{};
-        if (tune_lookup(key, &r)) {
+        // a kept result is applied only when its shape is one the trials could pick (the
+        // file may have been edited): lanes a power of two up to 32, the build interval 1,
+        // 2 or 5, a positive skin; gravity 1, 2 or 4 targets per thread and 1-64 chunks
+        const auto valid = [grav](const tune_result& q) {
+            const int a = (int)q.shape0, e = (int)q.every;
+            if (grav) {
+                const int ch = (int)q.shape1;
+                return (double)a == q.shape0 && (a == 1 || a == 2 || a == 4) &&
+                       (double)ch == q.shape1 && ch >= 1 && ch <= 64;
+            }
+            return (double)a == q.shape0 && a >= 1 && a <= 32 && (a & (a - 1)) == 0 &&
+                   std::isfinite(q.shape1) && q.shape1 > 0.0 && (double)e == q.every &&
+                   (e == 1 || e == 2 || e == 5);
+        };
+        if (tune_lookup(key, &r) && valid(r)) {
             if (grav) {
                 c->grav_tpt_set = (int)r.shape0;
                 c->grav_chunks_set = (int)r.shape1;

@@ -564,6 +564,8 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     }
     __syncthreads();
     const size_t stride = (size_t)n * G;
+    const int lgG = __builtin_ctz((unsigned)G);   // G is a power of two: slot r of atom i
+                                                  // is at (r >> lgG) stride + i G + r % G
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int own0 = hb; own0 < he; own0 += BUILD_OWN) {
         const int mo = (he - own0) < BUILD_OWN ? (he - own0) : BUILD_OWN;
@@ -688,7 +690,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                         if (hit) {
                             const int r = cc[k] + __popcll(mask & below);
                             if (r < cap)
-                                nbr[(size_t)(r / G) * stride + (size_t)ii[k] * G + (r % G)] = j;
+                                nbr[(size_t)(r >> lgG) * stride + (size_t)ii[k] * G + (r & (G - 1))] = j;
                         }
                         cc[k] += __popcll(mask);
                     }
@@ -745,6 +747,7 @@ __device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ 
     if (i == 0) atomicAdd(&flags[2], 1);
     const double4 pi = pos[i];
     const size_t stride = (size_t)n * G;
+    const int lgG = __builtin_ctz((unsigned)G);
     int c = 0;
     for (int j = 0; j < n; ++j) {
         if (j == i) continue;
@@ -755,7 +758,7 @@ __device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ 
         dy -= L * rint(dy * invL);
         dz -= L * rint(dz * invL);
         if (dx * dx + dy * dy + dz * dz < rlist2) {
-            if (c < cap) nbr[(size_t)(c / G) * stride + (size_t)i * G + (c % G)] = j;
+            if (c < cap) nbr[(size_t)(c >> lgG) * stride + (size_t)i * G + (c & (G - 1))] = j;
             ++c;
         }
     }
@@ -1011,9 +1014,10 @@ __global__ void k_count_pairs_list(const double4* __restrict__ pos, const int* _
     if (i >= n) return;
     const double4 pi = pos[i];
     const size_t stride = (size_t)n * G;
+    const int lgG = __builtin_ctz((unsigned)G);
     unsigned long long cnt = 0;
     for (int k = 0; k < nnb[i]; ++k) {
-        const int j = nbr[(size_t)(k / G) * stride + (size_t)i * G + (k % G)];
+        const int j = nbr[(size_t)(k >> lgG) * stride + (size_t)i * G + (k & (G - 1))];
         const double4 pj = pos[j];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         dx -= L * rint(dx * invL);
