@@ -139,6 +139,9 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     free_nl(nl);
     nl.lanes = lanes;
     const int64_t n = c->n;
+    // the list kernels index one list row of n * lanes slots in 32 bits
+    if (n * (int64_t)std::max(lanes, 1) >= (int64_t)1 << 32)
+        return nbx_fail(c, NBX_ERR_CAPACITY, "n x lanes per atom must stay below 2^32");
     const double L = c->bc.b[0];
     nl.cutoff = p.cutoff;
     nl.skin = c->skin_set ? c->skin : 0.12 * p.cutoff;

@@ -502,9 +502,8 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
               int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
               int* __restrict__ flags, int* __restrict__ req, int clear_bins, int rotate,
               int parts) {
-    __shared__ int skey[BUILD_SLOTS];
+    __shared__ __attribute__((aligned(16))) int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
-    __shared__ unsigned char sseg[BUILD_SLOTS];
     __shared__ int seg[BUILD_STENCIL + 1], segcell[BUILD_STENCIL];
     __shared__ int ncnt[BUILD_STENCIL], ncell[BUILD_STENCIL];
     __shared__ float4 nshift[BUILD_STENCIL], segshift[BUILD_STENCIL];
@@ -563,10 +562,9 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
         if (2 * lane + 1 <= nst) npre[2 * lane + 1] = ex + a0;
     }
     __syncthreads();
-    const size_t stride = (size_t)n * G;
-    const int lgG = __builtin_ctz((unsigned)G);   // G is a power of two: slot r of atom i
-                                                  // is at (r >> lgG) stride + i G + r % G
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // G is a power of two: slot r of atom i is at (r >> lgG) n G + i G + r % G
+    const uint32_t stride32 = (uint32_t)n * (uint32_t)G;
+    const int lgG = __builtin_ctz((unsigned)G);
     for (int own0 = hb; own0 < he; own0 += BUILD_OWN) {
         const int mo = (he - own0) < BUILD_OWN ? (he - own0) : BUILD_OWN;
         for (int t = tid; t < mo; t += BUILD_THREADS) {
@@ -603,49 +601,60 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             if (tid == 0) seg[nseg] = M;
             nb = lo;
             __syncthreads();
-            for (int t = tid; t < M; t += BUILD_THREADS) {
-                const int s = build_segment(seg, nseg, t);
-                sseg[t] = (unsigned char)s;
-                skey[t] = atoms[(size_t)segcell[s] * ccap + (t - seg[s])];
-            }
-            __syncthreads();
-            // positions of the candidates at their image next to the home cell (FP64 sum,
-            // one FP32 rounding) and their rank by index within their cell; then both are
-            // stored in that order, so the tests below read candidates in list order
-            float4 cp[BUILD_SLOTS / BUILD_THREADS];
-            int ck[BUILD_SLOTS / BUILD_THREADS], cr[BUILD_SLOTS / BUILD_THREADS];
-            float4 csh[BUILD_SLOTS / BUILD_THREADS];
+            // candidate t = tid + q BUILD_THREADS: its stencil cell (segment), index and
+            // position; the position gathers are issued here and stay in flight through
+            // the barrier and the rank counts below
+            constexpr int QN = BUILD_SLOTS / BUILD_THREADS;
+            int ck[QN], cs[QN], cr[QN];
+            double4 pg[QN];
 #pragma unroll
-            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+            for (int q = 0; q < QN; ++q) {
                 const int t = tid + q * BUILD_THREADS;
-                ck[q] = t < M ? skey[t] : -1;
-                csh[q] = (SHIFT && t < M) ? segshift[sseg[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-            // the gathers of all four candidates are in flight together
-#pragma unroll
-            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
-                if (ck[q] >= 0) {
-                    const double4 p = pos[ck[q]];
-                    cp[q] = make_float4((float)(p.x + (double)csh[q].x * Ld),
-                                        (float)(p.y + (double)csh[q].y * Ld),
-                                        (float)(p.z + (double)csh[q].z * Ld), 0.0f);
+                ck[q] = -1;
+                cs[q] = 0;
+                if (t < M) {
+                    const int s = build_segment(seg, nseg, t);
+                    cs[q] = s;
+                    ck[q] = atoms[(size_t)segcell[s] * ccap + (t - seg[s])];
+                    skey[t] = ck[q];
                 }
             }
 #pragma unroll
-            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
-                const int t = tid + q * BUILD_THREADS;
-                if (t < M) {
+            for (int q = 0; q < QN; ++q)
+                if (ck[q] >= 0) pg[q] = pos[ck[q]];
+            __syncthreads();
+            // rank by index within the candidate's cell (the cell's indices read four at
+            // a time where aligned), so that candidates are stored in list order
+#pragma unroll
+            for (int q = 0; q < QN; ++q) {
+                if (ck[q] >= 0) {
                     const int key = ck[q];
-                    const int s = sseg[t];
-                    const int b = seg[s], e1 = seg[s + 1];
-                    int r = 0;
-                    for (int u = b; u < e1; ++u) r += skey[u] < key;
+                    const int b = seg[cs[q]], e1 = seg[cs[q] + 1];
+                    int r = 0, u = b;
+                    for (; u < e1 && (u & 3) != 0; ++u) r += skey[u] < key;
+                    for (; u + 4 <= e1; u += 4) {
+                        const int4 k4 = *reinterpret_cast<const int4*>(&skey[u]);
+                        r += (k4.x < key) + (k4.y < key) + (k4.z < key) + (k4.w < key);
+                    }
+                    for (; u < e1; ++u) r += skey[u] < key;
                     cr[q] = b + r;
                 }
             }
+            // positions at the candidate's image next to the home cell (FP64 sum, one FP32
+            // rounding)
+            float4 cp[QN];
+#pragma unroll
+            for (int q = 0; q < QN; ++q) {
+                if (ck[q] >= 0) {
+                    const float4 sh = SHIFT ? segshift[cs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    cp[q] = make_float4((float)(pg[q].x + (double)sh.x * Ld),
+                                        (float)(pg[q].y + (double)sh.y * Ld),
+                                        (float)(pg[q].z + (double)sh.z * Ld), 0.0f);
+                }
+            }
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < BUILD_SLOTS / BUILD_THREADS; ++q) {
+            for (int q = 0; q < QN; ++q) {
                 if (ck[q] >= 0) {
                     skey[cr[q]] = ck[q];
                     spos[cr[q]] = cp[q];
@@ -658,16 +667,18 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             // and those tests are independent
             const int per = min(BUILD_OWN_BATCH, (mo + BUILD_THREADS / 64 - 1) / (BUILD_THREADS / 64));
             for (int o0 = wave * per; o0 < mo; o0 += BUILD_THREADS / 64 * per) {
-                const int no = min(per, mo - o0);
+                // the batch's own atoms are the same for the whole wave: their index and list
+                // count are held in scalar registers
+                const int no = __builtin_amdgcn_readfirstlane(min(per, mo - o0));
                 float4 pi[BUILD_OWN_BATCH];
                 int ii[BUILD_OWN_BATCH], im[BUILD_OWN_BATCH], cc[BUILD_OWN_BATCH];
 #pragma unroll
                 for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
                     const int o = k < no ? o0 + k : o0;
                     pi[k] = opos[o];
-                    ii[k] = okey[o];
+                    ii[k] = __builtin_amdgcn_readfirstlane(okey[o]);
                     im[k] = WATER ? ii[k] / 3 : 0;
-                    cc[k] = ocnt[o];
+                    cc[k] = __builtin_amdgcn_readfirstlane(ocnt[o]);
                 }
                 for (int tb = 0; tb < M; tb += 64) {
                     const int t = tb + lane;
@@ -677,21 +688,27 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                     const int jm = WATER ? j / 3 : 0;
 #pragma unroll
                     for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
-                        if (k >= no) break;   // uniform over the wave
+                        if (k >= no) break;   // scalar
                         float dx = pi[k].x - pj.x, dy = pi[k].y - pj.y, dz = pi[k].z - pj.z;
                         if (!SHIFT) {
                             dx -= Lf * rintf(dx * invLf);
                             dy -= Lf * rintf(dy * invLf);
                             dz -= Lf * rintf(dz * invLf);
                         }
-                        const bool hit = v && dx * dx + dy * dy + dz * dz < rlist2f && j != ii[k] &&
-                                         !(WATER && jm == im[k]);
+                        // every term evaluated (no short-circuit branches)
+                        const bool hit = v & (dx * dx + dy * dy + dz * dz < rlist2f) &
+                                         (j != ii[k]) & !(WATER && jm == im[k]);
                         const unsigned long long mask = __ballot(hit);
-                        if (hit) {
-                            const int r = cc[k] + __popcll(mask & below);
-                            if (r < cap)
-                                nbr[(size_t)(r >> lgG) * stride + (size_t)ii[k] * G + (r & (G - 1))] = j;
-                        }
+                        if (mask == 0ull) continue;   // scalar
+                        // slot r of a hit: the hits before this lane in the wave, after
+                        // the atom's earlier hits (slots past the capacity are counted, not
+                        // stored: the count flags the overflow)
+                        const unsigned lo = (unsigned)mask, hi = (unsigned)(mask >> 32);
+                        const int r = cc[k] + (int)__builtin_amdgcn_mbcnt_hi(
+                                                  hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                        if (hit & (r < cap))
+                            nbr[(uint64_t)(uint32_t)(r >> lgG) * stride32 +
+                                (uint32_t)(ii[k] * G + (r & (G - 1)))] = j;
                         cc[k] += __popcll(mask);
                     }
                 }
