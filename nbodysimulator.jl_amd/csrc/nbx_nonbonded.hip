@@ -369,16 +369,38 @@ void mx_launch_positions(hipStream_t s, const nbx_ctx* c, const double* x, void*
 }
 
 // bins of the wrapped positions (only when a list build is requested for them)
-__global__ void k_bin(const double4* __restrict__ pos, int n, int nc, double inv_cs,
-                      int* __restrict__ bin_count, int* __restrict__ bin_atoms, int ccap,
-                      int* __restrict__ flags, const int* __restrict__ req) {
+// Consecutive atoms of a workgroup that fall into the same cell (runs: the state is kept
+// in cell order during long runs) take their slots with one device-scope atomic per run
+// instead of one per atom; the order of atoms within a cell does not matter (the builds
+// rank the candidates by index).
+#define BIN_THREADS 256
+__global__ void __launch_bounds__(BIN_THREADS)
+k_bin(const double4* __restrict__ pos, int n, int nc, double inv_cs,
+      int* __restrict__ bin_count, int* __restrict__ bin_atoms, int ccap,
+      int* __restrict__ flags, const int* __restrict__ req) {
     if (req[0] == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double4 p = pos[i];
-    const int cid = (mx_cell_coord(p.z, inv_cs, nc) * nc + mx_cell_coord(p.y, inv_cs, nc)) * nc +
-                    mx_cell_coord(p.x, inv_cs, nc);
-    const int slot = atomicAdd(&bin_count[cid], 1);
+    __shared__ int scid[BIN_THREADS], sbase[BIN_THREADS];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * BIN_THREADS + t;
+    int cid = -1;
+    if (i < n) {
+        const double4 p = pos[i];
+        cid = (mx_cell_coord(p.z, inv_cs, nc) * nc + mx_cell_coord(p.y, inv_cs, nc)) * nc +
+              mx_cell_coord(p.x, inv_cs, nc);
+    }
+    scid[t] = cid;
+    __syncthreads();
+    // the run of equal cells that this thread belongs to: [head, end)
+    int head = t;
+    while (head > 0 && scid[head - 1] == cid) --head;
+    if (cid >= 0 && head == t) {
+        int end = t + 1;
+        while (end < BIN_THREADS && scid[end] == cid) ++end;
+        sbase[t] = atomicAdd(&bin_count[cid], end - t);
+    }
+    __syncthreads();
+    if (cid < 0) return;
+    const int slot = sbase[head] + (t - head);
     if (slot < ccap) bin_atoms[(size_t)cid * ccap + slot] = i;
     else flags[1] = 1;
 }
@@ -387,7 +409,8 @@ void mx_launch_bin(hipStream_t s, const nbx_ctx* c, const void* pos, const int* 
                    int* bin_count, int* bin_atoms) {
     const int nc = c->nl.ncell;
     if (c->n <= 0 || nc < 3) return;
-    hipLaunchKernelGGL(k_bin, dim3(nbx_blocks(c->n, 256)), dim3(256), 0, s, (const double4*)pos,
+    hipLaunchKernelGGL(k_bin, dim3(nbx_blocks(c->n, BIN_THREADS)), dim3(BIN_THREADS), 0, s,
+                       (const double4*)pos,
                        (int)c->n, nc, (double)nc / c->bc.b[0], bin_count, bin_atoms, c->nl.ccap,
                        c->nl.d_flags, req);
 }
