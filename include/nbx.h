@@ -236,7 +236,8 @@ int nbx_autotune(nbx_ctx* ctx, double dt, int32_t trial_steps, double* out, int3
  * so far, [1] list slots per atom, [2] cells per box side (0: all-pairs search),
  * [3] lanes per atom in the force kernel, [4] steps per captured graph, [5] steps between
  * list-build opportunities, [6] chunks replayed because a list went stale between two
- * opportunities (each such replay sets [5] to 1).  nout <= 7. */
+ * opportunities (each such replay sets [5] to 1), [7] spatial re-sorts of the state
+ * applied inside nbx_vv_run (NBX_SORT_EVERY).  nout <= 8. */
 int nbx_get_stats(nbx_ctx* ctx, int64_t* out, int32_t nout);
 
 /* ---------------------------------------------------------------------------------

@@ -28,6 +28,7 @@ ARCH = "gfx950"
 SOURCES = [
     ("nbx_ref64.hip", ["-ffp-contract=off"]),
     ("nbx_nonbonded.hip", []),
+    ("nbx_sort.hip", []),
     ("nbx_gravity.hip", []),
     ("nbx_batch.hip", []),
     ("nbx_batch.cpp", []),

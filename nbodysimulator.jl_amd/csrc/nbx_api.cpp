@@ -48,7 +48,10 @@ static void release(nbx_ctx* c) {
     if (c->d_perm) (void)hipFree(c->d_perm);
     if (c->d_iperm) (void)hipFree(c->d_iperm);
     if (c->d_pmap) (void)hipFree(c->d_pmap);
+    if (c->d_sortbuf) (void)hipFree(c->d_sortbuf);
     c->d_perm = c->d_iperm = c->d_pmap = nullptr;
+    c->d_sortbuf = nullptr;
+    c->sortbuf_bytes = 0;
     c->d_mm = nullptr;
     c->d_mm4 = nullptr;
     c->d_x = c->d_v = c->d_a = c->d_vh = c->d_tmp = c->d_ms = c->d_qs = nullptr;
@@ -123,13 +126,11 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_perm, bi);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_iperm, bi);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_pmap, bi);
-    if (e == hipSuccess) {
-        c->h_perm.resize((size_t)n);
-        for (int64_t k = 0; k < n; ++k) c->h_perm[(size_t)k] = (int)k;
-        c->h_iperm = c->h_perm;
-        if (n > 0) e = hipMemcpy(c->d_perm, c->h_perm.data(), bi, hipMemcpyHostToDevice);
-        if (e == hipSuccess && n > 0)
-            e = hipMemcpy(c->d_iperm, c->h_perm.data(), bi, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n > 0) {
+        std::vector<int> iota((size_t)n);
+        for (int64_t k = 0; k < n; ++k) iota[(size_t)k] = (int)k;
+        e = hipMemcpy(c->d_perm, iota.data(), bi, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_iperm, iota.data(), bi, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && n > 0)
         e = hipMemcpy(c->d_ms, masses, b1, hipMemcpyHostToDevice);
@@ -972,9 +973,9 @@ extern "C" int nbx_autotune(nbx_ctx* c, double dt, int32_t trial_steps, double* 
 }
 
 extern "C" int nbx_get_stats(nbx_ctx* c, int64_t* out, int32_t nout) {
-    if (!c || !out || nout < 0 || nout > 7) return NBX_ERR_INVALID;
-    int64_t v[7] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps, c->build_every,
-                    c->stale_replays};
+    if (!c || !out || nout < 0 || nout > 8) return NBX_ERR_INVALID;
+    int64_t v[8] = {0, c->nl.cap, c->nl.ncell, c->nl.lanes, c->graph_steps, c->build_every,
+                    c->stale_replays, c->sorts_applied};
     if (c->nl.d_flags) {
         int h[8] = {0};
         NBX_HIP(c, hipMemcpyAsync(h, c->nl.d_flags, sizeof(h), hipMemcpyDeviceToHost, c->stream));

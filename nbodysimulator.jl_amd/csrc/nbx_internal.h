@@ -163,10 +163,12 @@ struct nbx_ctx {
     int* d_perm = nullptr;    // slot -> user atom (identity when perm_identity)
     int* d_iperm = nullptr;   // user atom -> slot
     int* d_pmap = nullptr;    // scratch: a permutation being applied
-    std::vector<int> h_perm, h_iperm;
+    void* d_sortbuf = nullptr;   // the next order's maps, then the radix sort's scratch
+    size_t sortbuf_bytes = 0;
     bool perm_identity = true;
     bool list_sorted = false;
     int64_t last_sort_step = -1;
+    int64_t sorts_applied = 0;   // re-sorts that changed the order (diagnostics)
     std::string err;
 };
 

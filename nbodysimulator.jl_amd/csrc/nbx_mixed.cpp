@@ -483,16 +483,10 @@ static bool sort_enabled(const nbx_ctx* c, const mx_plan& p) {
 }
 
 int mx_reset_order(nbx_ctx* c) {
-    const int64_t n = c->n;
-    for (int64_t k = 0; k < n; ++k) c->h_perm[(size_t)k] = (int)k;
-    c->h_iperm = c->h_perm;
-    if (n > 0) {
-        NBX_HIP(c, hipMemcpyAsync(c->d_perm, c->h_perm.data(), sizeof(int) * n,
-                                  hipMemcpyHostToDevice, c->stream));
-        NBX_HIP(c, hipMemcpyAsync(c->d_iperm, c->h_perm.data(), sizeof(int) * n,
-                                  hipMemcpyHostToDevice, c->stream));
-        NBX_HIP(c, hipStreamSynchronize(c->stream));
-    }
+    mx_launch_iota(c->stream, c->d_perm, c->n);
+    mx_launch_iota(c->stream, c->d_iperm, c->n);
+    NBX_HIP(c, hipGetLastError());
+    NBX_HIP(c, hipStreamSynchronize(c->stream));
     c->perm_identity = true;
     c->list_sorted = false;
     c->last_sort_step = -1;
@@ -537,44 +531,52 @@ static int rebuild_list(nbx_ctx* c, mx_plan& p) {
 }
 
 // a new sorted order from the current positions: slots by cell of the list grid
-// (z-major), the atoms of a cell by user index; the state is reordered and the list
-// rebuilt.  The state must be in the current sorted order.
+// (z-major), the atoms of a cell by user index, found on the device (one radix sort); the
+// state is reordered and the list rebuilt.  A state whose key sequence steps back at most
+// n / 16 times is already nearly in cell order (an FCC lattice in generation order: about
+// n / 80) and is kept.  The state must be in the current sorted order.
 static int resort(nbx_ctx* c, mx_plan& p) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
-    std::vector<double> x(3 * (size_t)n);
-    NBX_HIP(c, hipMemcpyAsync(x.data(), c->d_x, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
-    NBX_HIP(c, hipStreamSynchronize(s));
-    const double L = c->bc.b[0];
     const int g = c->nl.ncell;
-    std::vector<std::pair<int64_t, int>> key((size_t)n);
-    for (int64_t sl = 0; sl < n; ++sl) {
-        int64_t cell = 0;
-        for (int k = 2; k >= 0; --k) {
-            const double xk = x[(size_t)(k * n + sl)];
-            const double w = xk - L * std::floor(xk / L);
-            int ci = (int)(w / L * g);
-            ci = ci < 0 ? 0 : (ci >= g ? g - 1 : ci);
-            cell = cell * g + ci;
-        }
-        key[(size_t)sl] = {cell * n + c->h_perm[(size_t)sl], (int)sl};
+    const size_t temp = mx_sort_temp_bytes(n, g);
+    if (temp == 0) return nbx_fail(c, NBX_ERR_DEVICE, "sort scratch size");
+    // the next order's two maps (8 n bytes), the descent count, then the sort's scratch at
+    // a 256-byte boundary
+    const size_t maps = (8 * (size_t)n + 4 + 255) & ~(size_t)255;
+    if (c->sortbuf_bytes < maps + temp) {
+        if (c->d_sortbuf) (void)hipFree(c->d_sortbuf);
+        c->d_sortbuf = nullptr;
+        c->sortbuf_bytes = 0;
+        NBX_HIP(c, hipMalloc(&c->d_sortbuf, maps + temp));
+        c->sortbuf_bytes = maps + temp;
     }
-    std::sort(key.begin(), key.end());
-    std::vector<int> map((size_t)n), perm((size_t)n), iperm((size_t)n);
-    for (int64_t t = 0; t < n; ++t) {
-        map[(size_t)t] = key[(size_t)t].second;                 // new slot t <- old slot
-        perm[(size_t)t] = c->h_perm[(size_t)key[(size_t)t].second];
-    }
-    for (int64_t t = 0; t < n; ++t) iperm[(size_t)perm[(size_t)t]] = (int)t;
-    int rc;
-    NBX_HIP(c, hipMemcpyAsync(c->d_pmap, map.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    if ((rc = permute_state(c, c->d_pmap))) return rc;
-    NBX_HIP(c, hipMemcpyAsync(c->d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    NBX_HIP(c, hipMemcpyAsync(c->d_iperm, iperm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    int* perm_new = static_cast<int*>(c->d_sortbuf);
+    int* iperm_new = perm_new + n;
+    // keys and values in the 24 n bytes of d_tmp: keys [0, 8n), sorted keys [8n, 16n),
+    // slots [16n, 20n); the sorted slots (the map) go to d_pmap
+    char* tb = reinterpret_cast<char*>(c->d_tmp);
+    auto* keys_in = reinterpret_cast<unsigned long long*>(tb);
+    auto* keys_out = reinterpret_cast<unsigned long long*>(tb + 8 * (size_t)n);
+    int* vals_in = reinterpret_cast<int*>(tb + 16 * (size_t)n);
+    unsigned* d_desc = reinterpret_cast<unsigned*>(iperm_new + n);
+    if (mx_launch_sort_keys(s, c->d_x, n, c->bc.b[0], g, c->d_perm, keys_in, vals_in, d_desc))
+        return nbx_fail(c, NBX_ERR_DEVICE, "device sort keys failed");
+    unsigned desc = 0;
+    NBX_HIP(c, hipMemcpyAsync(&desc, d_desc, sizeof(desc), hipMemcpyDeviceToHost, s));
     NBX_HIP(c, hipStreamSynchronize(s));
-    c->h_perm.swap(perm);
-    c->h_iperm.swap(iperm);
+    if ((int64_t)desc * 16 <= n) return NBX_OK;
+    if (mx_launch_sort_apply(s, n, g, keys_in, keys_out, vals_in, c->d_pmap,
+                             static_cast<char*>(c->d_sortbuf) + maps, temp, c->d_perm, perm_new,
+                             iperm_new))
+        return nbx_fail(c, NBX_ERR_DEVICE, "device sort of the state failed");
+    int rc;
+    if ((rc = permute_state(c, c->d_pmap))) return rc;
+    // the maps keep their addresses (captured step graphs read them)
+    NBX_HIP(c, hipMemcpyAsync(c->d_perm, perm_new, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    NBX_HIP(c, hipMemcpyAsync(c->d_iperm, iperm_new, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     c->perm_identity = false;
+    ++c->sorts_applied;
     return rebuild_list(c, p);
 }
 

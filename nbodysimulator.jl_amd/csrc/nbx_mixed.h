@@ -342,6 +342,18 @@ void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
 // just launched (and the Nose-Hoover update)
 void mx_launch_temperature_finish(hipStream_t s, const mx_io& io);
 // dst[k * n + s] = src[k * n + map[s]] for k < comps (map: a permutation of [0, n))
+// device-side sort of the state's slots by cell (nbx_sort.hip): scratch bytes of the radix
+// sort for n slots on a g^3 grid; the keys (cell * n + user index) and slots of the
+// current order with the number of descents of the key sequence; and the sort itself
+// (map[t] = old slot of new slot t; perm_new / iperm_new = the new slot <-> user maps)
+size_t mx_sort_temp_bytes(int64_t n, int g);
+int mx_launch_sort_keys(hipStream_t s, const double* x, int64_t n, double L, int g,
+                        const int* perm, unsigned long long* keys, int* vals,
+                        unsigned* descents);
+int mx_launch_sort_apply(hipStream_t s, int64_t n, int g, unsigned long long* keys_in,
+                         unsigned long long* keys_out, int* vals_in, int* map, void* temp,
+                         size_t temp_bytes, const int* perm, int* perm_new, int* iperm_new);
+void mx_launch_iota(hipStream_t s, int* p, int64_t n);
 void mx_launch_gather(hipStream_t s, const double* src, double* dst, const int* map,
                       int64_t n, int comps);
 // Verlet list (gated by req) from bins B; its last block clears req and, with

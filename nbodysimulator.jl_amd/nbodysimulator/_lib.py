@@ -405,11 +405,11 @@ class Context:
                     list_builds=int(r[7]), trip_rel_dev=r[8])
 
     def stats(self) -> dict:
-        out = (C.c_int64 * 7)()
-        self._check(self._lib.nbx_get_stats(self._h, out, 7))
+        out = (C.c_int64 * 8)()
+        self._check(self._lib.nbx_get_stats(self._h, out, 8))
         return dict(list_builds=out[0], list_slots=out[1], cells_per_side=out[2],
                     lanes=out[3], graph_steps=out[4], build_every=out[5],
-                    stale_replays=out[6])
+                    stale_replays=out[6], resorts=out[7])
 
 
 class Batch:
