@@ -960,12 +960,15 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
         double cosine = (rba[0] * rbc[0] + rba[1] * rbc[1] + rba[2] * rbc[2]) / (nba * nbc);
         cosine = cosine > 1.0 ? 1.0 : (cosine < -1.0 ? -1.0 : cosine);
         const double force = -ka * (acos(cosine) - aHOH);
+        // the common factors divided once (the kernel is one dependent FP64 chain per
+        // thread: each IEEE division is about 30 instructions on it)
+        const double sa = ia * force / nba, sc = ic * force / nbc;
         for (int k = 0; k < 3; ++k) {
-            const double fa = ia * ta[k] * force / nba;
-            const double fc = ic * tc[k] * force / nbc;
-            if (w == 1) acc[k] += fa / mi;
-            else if (w == 0) acc[k] += -(fa + fc) / mi;
-            else acc[k] += fc / mi;
+            const double fa = sa * ta[k];
+            const double fc = sc * tc[k];
+            if (w == 1) acc[k] += fa * im;
+            else if (w == 0) acc[k] -= (fa + fc) * im;
+            else acc[k] += fc * im;
         }
         if (mx_therm_force(kp)) {
             const double cb = mx_berendsen_coeff(kp);
