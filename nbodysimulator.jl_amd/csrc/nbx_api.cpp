@@ -49,8 +49,10 @@ static void release(nbx_ctx* c) {
     if (c->d_iperm) (void)hipFree(c->d_iperm);
     if (c->d_pmap) (void)hipFree(c->d_pmap);
     if (c->d_sortbuf) (void)hipFree(c->d_sortbuf);
+    if (c->d_permbuf) (void)hipFree(c->d_permbuf);
     c->d_perm = c->d_iperm = c->d_pmap = nullptr;
     c->d_sortbuf = nullptr;
+    c->d_permbuf = nullptr;
     c->sortbuf_bytes = 0;
     c->d_mm = nullptr;
     c->d_mm4 = nullptr;

@@ -165,6 +165,7 @@ struct nbx_ctx {
     int* d_pmap = nullptr;    // scratch: a permutation being applied
     void* d_sortbuf = nullptr;   // the next order's maps, then the radix sort's scratch
     size_t sortbuf_bytes = 0;
+    double* d_permbuf = nullptr; // scratch of a reordering: 17 n doubles
     bool perm_identity = true;
     bool list_sorted = false;
     int64_t last_sort_step = -1;

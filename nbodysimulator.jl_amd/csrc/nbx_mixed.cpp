@@ -446,14 +446,21 @@ static int grow_lists(nbx_ctx* c, mx_plan& p, double& grow) {
     return prepare(c, p, grow);
 }
 
+static int rebuild_list(nbx_ctx* c, mx_plan& p);
+
 // Lists and accelerations ready for the current state.  Outside a run (run = false) the
-// state is in the user's order, so a list built for a run's sorted order is rebuilt.
+// state is in the user's order, so a list built for a run's sorted order is rebuilt for
+// it (the accelerations were returned in the user's order with the state).
 static int ensure_ready(nbx_ctx* c, mx_plan& p, bool run = false) {
     int rc;
     double grow = 1.0;
     if (!run && c->list_sorted) {
-        c->accel_valid = false;
         c->list_sorted = false;
+        if (c->nl.ready && c->accel_valid) {
+            std::string why;
+            if (!make_plan(c, p, &why)) return nbx_fail(c, NBX_ERR_UNSUPPORTED, why);
+            return rebuild_list(c, p);
+        }
     }
     if (!c->nl.ready || !c->accel_valid) {
         c->list_sorted = false;   // the list made here is for the state's current order
@@ -494,22 +501,21 @@ int mx_reset_order(nbx_ctx* c) {
 }
 
 // the per-atom state reordered in place: new[s] = old[map[s]] (positions, velocities,
-// accelerations, half-kicked velocities, list reference positions, masses, charges)
+// accelerations, half-kicked velocities, list reference positions, masses, charges), in
+// two launches through a scratch buffer
 static int permute_state(nbx_ctx* c, const int* d_map) {
-    hipStream_t s = c->stream;
     const int64_t n = c->n;
+    if (!c->d_permbuf)
+        NBX_HIP(c, hipMalloc((void**)&c->d_permbuf, sizeof(double) * 17 * (size_t)std::max<int64_t>(n, 1)));
+    mx_perm_arrays A{};
     double* a3[5] = {c->d_x, c->d_v, c->d_a, c->d_vh, c->nl.d_xref};
-    for (double* a : a3) {
-        if (!a) continue;
-        mx_launch_gather(s, a, c->d_tmp, d_map, n, 3);
-        NBX_HIP(c, hipMemcpyAsync(a, c->d_tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
-    }
+    for (double* a : a3)
+        if (a) A.a3[A.n3++] = a;
     double* a1[2] = {c->d_ms, c->has_q ? c->d_qs : nullptr};
-    for (double* a : a1) {
-        if (!a) continue;
-        mx_launch_gather(s, a, c->d_tmp, d_map, n, 1);
-        NBX_HIP(c, hipMemcpyAsync(a, c->d_tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    }
+    for (double* a : a1)
+        if (a) A.a1[A.n1++] = a;
+    mx_launch_permute(c->stream, d_map, n, A, c->d_permbuf);
+    NBX_HIP(c, hipGetLastError());
     return NBX_OK;
 }
 
@@ -580,13 +586,15 @@ static int resort(nbx_ctx* c, mx_plan& p) {
     return rebuild_list(c, p);
 }
 
-int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every, double* uf,
-          double* vf, int64_t* nframes) {
+// allow_sort = false: internal callers that alternate short runs with work on the state
+// in the user's order (training-set collection, the learned stride's plain steps)
+static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every,
+                     double* uf, double* vf, int64_t* nframes, bool allow_sort) {
     mx_plan p;
     int rc;
     if ((rc = ensure_ready(c, p, true))) return rc;
     // the run keeps the state in the sorted order and returns it in the user's order
-    const bool sorting = sort_enabled(c, p);
+    const bool sorting = allow_sort && sort_enabled(c, p);
     struct order_guard {
         nbx_ctx* c = nullptr;
         ~order_guard() {
@@ -604,11 +612,11 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         } else {
             og.c = c;
         }
-    } else if (c->list_sorted) {
-        // a list of another run's sorted order (the plan changed since)
-        c->list_sorted = false;
-        c->accel_valid = false;
-        if ((rc = ensure_ready(c, p, true))) return rc;
+    } else if (!c->perm_identity || c->list_sorted) {
+        // not sorted from here on (an internal caller, or the plan no longer sorts): the
+        // state is in the user's order, the list was made for the sorted one
+        if ((rc = mx_reset_order(c))) return rc;
+        if ((rc = rebuild_list(c, p))) return rc;
     }
     double grow = 1.0;
     hipStream_t s = c->stream;
@@ -806,6 +814,11 @@ int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_eve
         }
     }
     return NBX_OK;
+}
+
+int mx_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t save_every, double* uf,
+          double* vf, int64_t* nframes) {
+    return mx_vv_run(c, dt, nsteps, seed, save_every, uf, vf, nframes, true);
 }
 
 // the step's dominant (force) kernel
@@ -1074,7 +1087,7 @@ int mx_learn_collect(nbx_ctx* c, nbx_model* m, double dt, int stride, int64_t nf
         lr_launch_features(s, c->nl.lanes, fp);
         NBX_HIP(c, hipGetLastError());
         int64_t nf = 0;
-        if ((rc = mx_vv(c, dt, stride, seed, 0, nullptr, nullptr, &nf))) return rc;
+        if ((rc = mx_vv_run(c, dt, stride, seed, 0, nullptr, nullptr, &nf, false))) return rc;
         lr_target_params tp{};
         tp.sel = sel.as<int>();
         tp.nsel = nsel;
@@ -1223,7 +1236,7 @@ int mx_learned_run(nbx_ctx* c, const nbx_model* m, double dt, int stride, int64_
         c->accel_valid = false;
         plain = (nbig - done) * stride;
         int64_t nf = 0;
-        if ((rc = mx_vv(c, dt, plain, 0, 0, nullptr, nullptr, &nf))) return rc;
+        if ((rc = mx_vv_run(c, dt, plain, 0, 0, nullptr, nullptr, &nf, false))) return rc;
         if ((rc = fresh_list(c, p))) return rc;
         if ((rc = list_energy(c, 0, lj, epart, &ke, &pe))) return rc;
         E = ke + pe;

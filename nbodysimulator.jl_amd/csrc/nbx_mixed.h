@@ -354,6 +354,15 @@ int mx_launch_sort_apply(hipStream_t s, int64_t n, int g, unsigned long long* ke
                          unsigned long long* keys_out, int* vals_in, int* map, void* temp,
                          size_t temp_bytes, const int* perm, int* perm_new, int* iperm_new);
 void mx_launch_iota(hipStream_t s, int* p, int64_t n);
+// arrays of n atoms reordered together: n3 of 3 SoA components, n1 of one
+struct mx_perm_arrays {
+    double* a3[5];
+    double* a1[2];
+    int n3, n1;
+};
+// new[s] = old[map[s]] for every array, through tmp ((3 n3 + n1) n doubles)
+void mx_launch_permute(hipStream_t s, const int* map, int64_t n, const mx_perm_arrays& A,
+                       double* tmp);
 void mx_launch_gather(hipStream_t s, const double* src, double* dst, const int* map,
                       int64_t n, int comps);
 // Verlet list (gated by req) from bins B; its last block clears req and, with

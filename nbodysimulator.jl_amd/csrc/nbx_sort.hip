@@ -112,3 +112,36 @@ void mx_launch_iota(hipStream_t s, int* p, int64_t n) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_iota, dim3(nbx_blocks(n, 256)), dim3(256), 0, s, p, n);
 }
+
+// the per-atom state reordered: new[s] = old[map[s]] for every array, through a scratch
+// buffer of (3 n3 + n1) n doubles (two launches for the whole state)
+__global__ void __launch_bounds__(256)
+k_perm_gather(const int* __restrict__ map, int64_t n, mx_perm_arrays A, double* __restrict__ tmp) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t j = map[s];
+    int o = 0;
+    for (int q = 0; q < A.n3; ++q, o += 3)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) tmp[(o + k) * n + s] = A.a3[q][k * n + j];
+    for (int q = 0; q < A.n1; ++q, ++o) tmp[o * n + s] = A.a1[q][j];
+}
+
+__global__ void __launch_bounds__(256)
+k_perm_store(int64_t n, mx_perm_arrays A, const double* __restrict__ tmp) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int o = 0;
+    for (int q = 0; q < A.n3; ++q, o += 3)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A.a3[q][k * n + s] = tmp[(o + k) * n + s];
+    for (int q = 0; q < A.n1; ++q, ++o) A.a1[q][s] = tmp[o * n + s];
+}
+
+void mx_launch_permute(hipStream_t s, const int* map, int64_t n, const mx_perm_arrays& A,
+                       double* tmp) {
+    if (n <= 0) return;
+    const unsigned blocks = nbx_blocks(n, 256);
+    hipLaunchKernelGGL(k_perm_gather, dim3(blocks), dim3(256), 0, s, map, n, A, tmp);
+    hipLaunchKernelGGL(k_perm_store, dim3(blocks), dim3(256), 0, s, n, A, tmp);
+}
