@@ -39,7 +39,9 @@ struct nbx_nlist {
     double rlist = 0.0;       // list radius = cutoff + skin
     double cutoff = 0.0;      // largest interaction cutoff
     double skin = 0.0;        // skin in use (rebuild when an atom moved skin/2)
-    int* d_nbr = nullptr;     // neighbour slots, interleaved by lanes (see nbx_nonbonded.hip)
+    void* d_nbr = nullptr;    // neighbour slots, interleaved by lanes (see nbx_nonbonded.hip):
+                              // 16-bit atom indices for n <= 65,536, else 32-bit
+    int nbr_wide = 1;         // 1: 32-bit entries, 0: 16-bit
     int* d_nnb = nullptr;     // neighbour counts
     int* d_cell_count = nullptr;  // 3 x ncells (NBX_BINS_STEP: sets of even / odd steps;
                                   // NBX_BINS_BUILD: three sets rotating at each build)

@@ -384,9 +384,9 @@ __global__ void __launch_bounds__(256) k_lr_features(lr_feat_params fp) {
 #pragma unroll
     for (int b = 0; b < LR_NBINS; ++b) g[b] = 0.0f;
     const size_t stride = (size_t)n * G;
-    const int* p = fp.nbr + (size_t)i * G + lane;
-    for (int k = lane; k < cnt; k += G, p += stride) {
-        const double4 pj = fp.pos[*p];
+    size_t e = (size_t)i * G + lane;
+    for (int k = lane; k < cnt; k += G, e += stride) {
+        const double4 pj = fp.pos[mx_nbr_load(fp.nbr, fp.nbr_wide, e)];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         dx -= fp.L * rint(dx * fp.invL);
         dy -= fp.L * rint(dy * fp.invL);
@@ -666,9 +666,9 @@ __global__ void __launch_bounds__(256) k_lr_energy(lr_energy_params ep) {
         const double4 pi = ep.pos[i];
         const int cnt = ep.nnb[i];
         const size_t stride = (size_t)n * G;
-        const int* p = ep.nbr + gid;
-        for (int k = lane; k < cnt; k += G, p += stride) {
-            const double4 pj = ep.pos[*p];
+        size_t e = (size_t)gid;
+        for (int k = lane; k < cnt; k += G, e += stride) {
+            const double4 pj = ep.pos[mx_nbr_load(ep.nbr, ep.nbr_wide, e)];
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             dx -= ep.L * rint(dx * ep.invL);
             dy -= ep.L * rint(dy * ep.invL);

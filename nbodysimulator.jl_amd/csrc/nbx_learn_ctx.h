@@ -6,7 +6,8 @@
 
 struct lr_feat_params {
     const double4* pos;   // wrapped positions (list mode)
-    const int* nbr;       // Verlet list (lane-interleaved)
+    const void* nbr;      // Verlet list (lane-interleaved; 16- or 32-bit entries)
+    int nbr_wide;         // 1: 32-bit entries
     const int* nnb;
     int n;
     double L, invL;
@@ -51,7 +52,8 @@ struct lr_target_params {
 
 struct lr_energy_params {
     const double4* pos;
-    const int* nbr;
+    const void* nbr;
+    int nbr_wide;
     const int* nnb;
     int n;
     double L, invL;

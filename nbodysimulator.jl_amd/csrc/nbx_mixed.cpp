@@ -119,7 +119,8 @@ void free_nl(nbx_nlist& nl) {
     if (nl.d_cell_atoms) (void)hipFree(nl.d_cell_atoms);
     if (nl.d_xref) (void)hipFree(nl.d_xref);
     if (nl.d_flags) (void)hipFree(nl.d_flags);
-    nl.d_nbr = nl.d_nnb = nl.d_cell_count = nl.d_cell_atoms = nl.d_flags = nullptr;
+    nl.d_nbr = nullptr;
+    nl.d_nnb = nl.d_cell_count = nl.d_cell_atoms = nl.d_flags = nullptr;
     nl.d_xref = nullptr;
     nl.ready = false;
 }
@@ -176,7 +177,9 @@ int alloc_lists(nbx_ctx* c, const mx_plan& p, double grow) {
     if (nl.ccap > 1024)
         return nbx_fail(c, NBX_ERR_CAPACITY, "cell occupancy above 1024 atoms (box too dense)");
     const size_t ncells = (size_t)nc * nc * nc;
-    NBX_HIP(c, hipMalloc((void**)&nl.d_nbr, sizeof(int) * (size_t)nl.cap * (size_t)n));
+    nl.nbr_wide = n > 65536 ? 1 : 0;
+    NBX_HIP(c, hipMalloc(&nl.d_nbr, (nl.nbr_wide ? sizeof(int) : sizeof(uint16_t)) *
+                                        (size_t)nl.cap * (size_t)n));
     NBX_HIP(c, hipMalloc((void**)&nl.d_nnb, sizeof(int) * (size_t)n));
     NBX_HIP(c, hipMalloc((void**)&nl.d_cell_count, sizeof(int) * 3 * std::max<size_t>(ncells, 1)));
     NBX_HIP(c, hipMalloc((void**)&nl.d_cell_atoms,
@@ -967,6 +970,7 @@ lr_feat_params feat_params(nbx_ctx* c, int par) {
     lr_feat_params fp{};
     fp.pos = (const double4*)pbuf(c, par);
     fp.nbr = c->nl.d_nbr;
+    fp.nbr_wide = c->nl.nbr_wide;
     fp.nnb = c->nl.d_nnb;
     fp.n = (int)c->n;
     fp.L = c->bc.b[0];
@@ -981,6 +985,7 @@ lr_energy_params energy_params(nbx_ctx* c, int par, const lj_consts& lj, double*
     lr_energy_params ep{};
     ep.pos = (const double4*)pbuf(c, par);
     ep.nbr = c->nl.d_nbr;
+    ep.nbr_wide = c->nl.nbr_wide;
     ep.nnb = c->nl.d_nnb;
     ep.n = (int)c->n;
     ep.L = c->bc.b[0];

@@ -181,6 +181,17 @@ __device__ __forceinline__ double mx_wrap(double p, double L) {
     return w >= L ? w - L : w;
 }
 
+// Verlet-list entries: 16-bit atom indices when the system has at most 65,536 atoms (half
+// the list bytes), 32-bit otherwise; `wide` is uniform over a launch
+__device__ __forceinline__ int mx_nbr_load(const void* nbr, int wide, size_t k) {
+    return wide ? static_cast<const int*>(nbr)[k]
+                : (int)static_cast<const uint16_t*>(nbr)[k];
+}
+__device__ __forceinline__ void mx_nbr_store(void* nbr, int wide, size_t k, int j) {
+    if (wide) static_cast<int*>(nbr)[k] = j;
+    else static_cast<uint16_t*>(nbr)[k] = (uint16_t)j;
+}
+
 // Fixed-point copy of the wrapped positions (list plans): the box fraction of each
 // coordinate as an unsigned 32-bit integer and the charge as FP32 bits, stored after the n
 // double4 of a position buffer.  The difference of two such words, read as a signed

@@ -13,7 +13,9 @@
 //                    gathers: the integer difference is the minimum image
 //   nbr            : Verlet list, interleaved for G lanes per atom: slot k of atom i is
 //                    at (k / G) * (n * G) + i * G + (k % G), so lane l of atom i reads
-//                    consecutive words with the other lanes of the wavefront
+//                    consecutive entries with the other lanes of the wavefront; entries
+//                    are 16-bit atom indices up to 65,536 atoms (half the bytes of the
+//                    list stream, no decoding), 32-bit above
 //   B[3]           : fixed-capacity cell bins (count + ccap slots per cell); a list is
 //                    rebuilt when an atom moved more than skin/2 since the last build:
 //                    the drift sets a request flag, and the build launch of the next
@@ -111,9 +113,9 @@ __device__ __forceinline__ double nb_finish(int64_t i, int64_t n, const double* 
 // Verlet-list force kernel: G lanes per atom
 // ---------------------------------------------------------------------------
 
-template <int G, bool LJ, int EL, bool WATER>
+template <int G, bool LJ, int EL, bool WATER, typename IDX>
 __global__ void __launch_bounds__(NB_THREADS)
-k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
+k_nb_list(const double4* __restrict__ pos, const IDX* __restrict__ nbr,
           const int* __restrict__ nnb, int n, mx_pair_params pp, mx_kick_params kp, mx_io io) {
     const int gid = mx_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     mx_resolve_step(kp);
@@ -137,7 +139,7 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[q].init();
         const size_t stride = (size_t)n * G;
-        const int* p = nbr + gid;
+        const IDX* p = nbr + gid;
         // NB_UNROLL slots per lane and pass: the position gathers of a pass are issued
         // together, and the slots of the next pass are loaded while this pass computes,
         // so a lane waits for about one gather per pass rather than two dependent loads
@@ -148,7 +150,7 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
         const float sc = (float)(pp.L * 0x1p-32);
         int jn[NB_UNROLL];
 #pragma unroll
-        for (int u = 0; u < NB_UNROLL; ++u) jn[u] = lane + u * G < cnt ? p[u * stride] : i;
+        for (int u = 0; u < NB_UNROLL; ++u) jn[u] = lane + u * G < cnt ? (int)p[u * stride] : i;
         for (int k = lane; k < cnt; k += NB_UNROLL * G) {
             int jj[NB_UNROLL];
             uint4 pq[NB_UNROLL];
@@ -159,7 +161,7 @@ k_nb_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
             p += NB_UNROLL * stride;
             const int kn = k + NB_UNROLL * G;
 #pragma unroll
-            for (int u = 0; u < NB_UNROLL; ++u) jn[u] = kn + u * G < cnt ? p[u * stride] : i;
+            for (int u = 0; u < NB_UNROLL; ++u) jn[u] = kn + u * G < cnt ? (int)p[u * stride] : i;
 #pragma unroll
             for (int u = 0; u < NB_UNROLL; ++u) {
                 if (k + u * G >= cnt) break;
@@ -245,9 +247,15 @@ template <int G, bool LJ, int EL, bool WATER>
 static void launch_list(hipStream_t s, const nbx_ctx* c, const void* pos,
                         const mx_pair_params& pp, const mx_kick_params& kp, const mx_io& io) {
     const int64_t threads = c->n * G;
-    hipLaunchKernelGGL((k_nb_list<G, LJ, EL, WATER>), dim3(nbx_blocks(threads, NB_THREADS)),
-                       dim3(NB_THREADS), 0, s, (const double4*)pos, c->nl.d_nbr, c->nl.d_nnb,
-                       (int)c->n, pp, kp, io);
+    if (c->nl.nbr_wide)
+        hipLaunchKernelGGL((k_nb_list<G, LJ, EL, WATER, int>), dim3(nbx_blocks(threads, NB_THREADS)),
+                           dim3(NB_THREADS), 0, s, (const double4*)pos,
+                           static_cast<const int*>(c->nl.d_nbr), c->nl.d_nnb, (int)c->n, pp, kp, io);
+    else
+        hipLaunchKernelGGL((k_nb_list<G, LJ, EL, WATER, uint16_t>),
+                           dim3(nbx_blocks(threads, NB_THREADS)), dim3(NB_THREADS), 0, s,
+                           (const double4*)pos, static_cast<const uint16_t*>(c->nl.d_nbr),
+                           c->nl.d_nnb, (int)c->n, pp, kp, io);
 }
 
 template <bool LJ, int EL, bool WATER>
@@ -522,9 +530,9 @@ __global__ void __launch_bounds__(BUILD_THREADS)
 k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int n, int nc,
               double Ld, int* __restrict__ bin_count, int* __restrict__ bin_atoms,
               int ccap, double inv_cs, float rlist2f, int G, int cap,
-              int* __restrict__ nbr, int* __restrict__ nnb, double* __restrict__ xref,
-              int* __restrict__ flags, int* __restrict__ req, int clear_bins, int rotate,
-              int parts) {
+              void* __restrict__ nbr, int nbr_wide, int* __restrict__ nnb,
+              double* __restrict__ xref, int* __restrict__ flags, int* __restrict__ req,
+              int clear_bins, int rotate, int parts) {
     __shared__ __attribute__((aligned(16))) int skey[BUILD_SLOTS];
     __shared__ float4 spos[BUILD_SLOTS];
     __shared__ int seg[BUILD_STENCIL + 1], segcell[BUILD_STENCIL];
@@ -730,8 +738,9 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                         const int r = cc[k] + (int)__builtin_amdgcn_mbcnt_hi(
                                                   hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
                         if (hit & (r < cap))
-                            nbr[(uint64_t)(uint32_t)(r >> lgG) * stride32 +
-                                (uint32_t)(ii[k] * G + (r & (G - 1)))] = j;
+                            mx_nbr_store(nbr, nbr_wide,
+                                         (uint64_t)(uint32_t)(r >> lgG) * stride32 +
+                                             (uint32_t)(ii[k] * G + (r & (G - 1))), j);
                         cc[k] += __popcll(mask);
                     }
                 }
@@ -780,8 +789,8 @@ template <bool WATER>
 __device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ pos,
                                                     const double* __restrict__ x, int i, int n,
                                                     double L, double invL, double rlist2, int G,
-                                                    int cap, int* __restrict__ nbr,
-                                                    int* __restrict__ nnb,
+                                                    int cap, void* __restrict__ nbr,
+                                                    int nbr_wide, int* __restrict__ nnb,
                                                     double* __restrict__ xref,
                                                     int* __restrict__ flags) {
     if (i == 0) atomicAdd(&flags[2], 1);
@@ -798,7 +807,8 @@ __device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ 
         dy -= L * rint(dy * invL);
         dz -= L * rint(dz * invL);
         if (dx * dx + dy * dy + dz * dz < rlist2) {
-            if (c < cap) nbr[(size_t)(c >> lgG) * stride + (size_t)i * G + (c & (G - 1))] = j;
+            if (c < cap)
+                mx_nbr_store(nbr, nbr_wide, (size_t)(c >> lgG) * stride + (size_t)i * G + (c & (G - 1)), j);
             ++c;
         }
     }
@@ -812,12 +822,14 @@ __device__ __forceinline__ void build_allpairs_atom(const double4* __restrict__ 
 template <bool WATER>
 __global__ void k_build_allpairs(const double4* __restrict__ pos, const double* __restrict__ x,
                                  int n, double L, double invL, double rlist2, int G, int cap,
-                                 int* __restrict__ nbr, int* __restrict__ nnb,
+                                 void* __restrict__ nbr, int nbr_wide, int* __restrict__ nnb,
                                  double* __restrict__ xref, int* __restrict__ flags,
                                  int* __restrict__ req) {
     if (req[0] == 0) return;   // uniform: only this kernel's last block clears the request
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) build_allpairs_atom<WATER>(pos, x, i, n, L, invL, rlist2, G, cap, nbr, nnb, xref, flags);
+    if (i < n)
+        build_allpairs_atom<WATER>(pos, x, i, n, L, invL, rlist2, G, cap, nbr, nbr_wide, nnb,
+                                   xref, flags);
     __shared__ int last;
     __syncthreads();
     if (threadIdx.x == 0)   // no fence: the last block only clears what every block read
@@ -838,7 +850,7 @@ static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, con
     const double L = c->bc.b[0];
     const int nc = nl.ncell;
     const double r2 = nl.rlist * nl.rlist;
-    int* nbr = nl.d_nbr;
+    void* nbr = nl.d_nbr;
     if (nc >= 3) {
         const int ncells = nc * nc * nc;
         const float r2f = (float)r2 * 1.0001f;
@@ -855,16 +867,18 @@ static void launch_build_t(hipStream_t s, const nbx_ctx* c, const void* pos, con
             hipLaunchKernelGGL((k_build_cells<WATER, true>), grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, L, bin_count,
                                bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
+                               nl.nbr_wide, nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins,
+                               rotate, parts);
         else
             hipLaunchKernelGGL((k_build_cells<WATER, false>), grid, dim3(BUILD_THREADS), 0, s,
                                (const double4*)pos, x, n, nc, L, bin_count,
                                bin_atoms, nl.ccap, (double)nc / L, r2f, nl.lanes, nl.cap, nbr,
-                               nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins, rotate, parts);
+                               nl.nbr_wide, nl.d_nnb, nl.d_xref, nl.d_flags, req, clear_bins,
+                               rotate, parts);
     } else {
         hipLaunchKernelGGL((k_build_allpairs<WATER>), dim3(nbx_blocks(n, 128)), dim3(128), 0,
                            s, (const double4*)pos, x, n, L, 1.0 / L, r2, nl.lanes, nl.cap, nbr,
-                           nl.d_nnb, nl.d_xref, nl.d_flags, req);
+                           nl.nbr_wide, nl.d_nnb, nl.d_xref, nl.d_flags, req);
     }
 }
 
@@ -1049,7 +1063,8 @@ void mx_launch_temperature(hipStream_t s, const nbx_ctx* c, const double* v, int
 // ---------------------------------------------------------------------------
 
 template <bool WATER>
-__global__ void k_count_pairs_list(const double4* __restrict__ pos, const int* __restrict__ nbr,
+__global__ void k_count_pairs_list(const double4* __restrict__ pos, const void* __restrict__ nbr,
+                                   int nbr_wide,
                                    const int* __restrict__ nnb, int n, int G, double L,
                                    double invL, double R2lj, double R2el, int has_lj,
                                    int has_el, unsigned long long* __restrict__ out) {
@@ -1060,7 +1075,7 @@ __global__ void k_count_pairs_list(const double4* __restrict__ pos, const int* _
     const int lgG = __builtin_ctz((unsigned)G);
     unsigned long long cnt = 0;
     for (int k = 0; k < nnb[i]; ++k) {
-        const int j = nbr[(size_t)(k >> lgG) * stride + (size_t)i * G + (k & (G - 1))];
+        const int j = mx_nbr_load(nbr, nbr_wide, (size_t)(k >> lgG) * stride + (size_t)i * G + (k & (G - 1)));
         const double4 pj = pos[j];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         dx -= L * rint(dx * invL);
@@ -1082,12 +1097,12 @@ void mx_launch_count_pairs(hipStream_t s, const nbx_ctx* c, const mx_nb_launch& 
     const int n = (int)c->n;
     if (L.water)
         hipLaunchKernelGGL(k_count_pairs_list<true>, dim3(nbx_blocks(n, 256)), dim3(256), 0, s,
-                           (const double4*)pos, c->nl.d_nbr, c->nl.d_nnb, n, c->nl.lanes,
+                           (const double4*)pos, c->nl.d_nbr, c->nl.nbr_wide, c->nl.d_nnb, n, c->nl.lanes,
                            pp.L, pp.invL, (double)pp.lj_R2, (double)pp.el_R2, L.has_lj,
                            L.el_kind != 0, d_count);
     else
         hipLaunchKernelGGL(k_count_pairs_list<false>, dim3(nbx_blocks(n, 256)), dim3(256), 0, s,
-                           (const double4*)pos, c->nl.d_nbr, c->nl.d_nnb, n, c->nl.lanes,
+                           (const double4*)pos, c->nl.d_nbr, c->nl.nbr_wide, c->nl.d_nnb, n, c->nl.lanes,
                            pp.L, pp.invL, (double)pp.lj_R2, (double)pp.el_R2, L.has_lj,
                            L.el_kind != 0, d_count);
 }

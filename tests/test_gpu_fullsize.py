@@ -93,6 +93,39 @@ def test_lj_32000(nbx):
     assert momentum_ratio(am, s.masses) <= 1e-5
 
 
+def test_lj_list_entries_of_both_widths(nbx):
+    """Verlet lists hold 16-bit atom indices up to 65,536 atoms and 32-bit ones above: an
+    FCC liquid of 4 x 26^3 = 70,304 atoms (32-bit) and the same density at 4 x 25^3 =
+    62,500 atoms (16-bit) against the oracle on sampled targets, with a short run whose
+    energy drift stays small (the lists of both widths through rebuilds)."""
+    lib = O.load()
+    for cells in (25, 26):
+        s = gen.lj_liquid(cells, rho=0.8442, T=0.9, seed=6, jitter=0.02)
+        n = s.u.shape[0]
+        print(f"[case] mixed LJ N={n} list entries {'32' if n > 65536 else '16'}-bit", flush=True)
+        targets = np.random.default_rng(cells).choice(n, 128, replace=False)
+        ref = np.zeros((len(targets), 3))
+        uu = np.ascontiguousarray(s.u)
+        idx = np.arange(n, dtype=np.int64)
+        bc = O.make_bc(O.BC_CUBIC, s.L)
+        for k, i in enumerate(targets):
+            lib.orc_lj_accel(O.dp(ref[k]), O.dp(uu), int(i), idx.ctypes.data_as(C.POINTER(C.c_int64)),
+                             n, O.dp(np.ones(n)), 1.0, 1.0, 2.5, C.byref(bc))
+        c = _lib.Context(n, _lib.SYS_POTENTIAL, s.masses)
+        c.set_boundary(_lib.BC_CUBIC, [s.L, 0, 0, 0, 0, 0])
+        c.add_lennard_jones(1.0, 1.0, 2.5)
+        c.set_kb(1.0)
+        c.set_precision(_lib.PREC_MIXED)
+        am = mixed_accel(c, s.u)
+        assert relerr(am[targets], ref) <= 2e-6
+        c.set_state(s.u, s.v)
+        e0 = sum(c.energies())
+        c.vv_run(0.005, 400)
+        e1 = sum(c.energies())
+        assert c.stats()["list_builds"] >= 2
+        assert abs(e1 - e0) <= 1e-4 * abs(e0)
+
+
 def test_water_4096(nbx):
     print("[case] full size water 4096 molecules", flush=True)
     w = gen.water_box(per_side=16, seed=7)
