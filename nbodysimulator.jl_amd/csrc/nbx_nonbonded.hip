@@ -495,9 +495,10 @@ void mx_launch_gather(hipStream_t s, const double* src, double* dst, const int* 
 //      by index within each cell, so every list comes out in a reproducible order (cell
 //      order, then index order);
 //   3. wave w takes own atoms 8 at a time; its 64 lanes read 64 candidates at a time and
-//      test each against the 8 (per-step bins: a plain difference, the images being the
-//      near ones; other bins: the minimum image), appending every atom's hits in candidate
-//      order (ballot + prefix count).
+//      test each against the 8 (per-step bins: a plain FP32 difference, the images being
+//      the near ones; other bins: the difference of the fixed-point box fractions, whose
+//      integer wrap-around is the minimum image), appending every atom's hits in
+//      candidate order (ballot + prefix count).
 //   The squared list radius carries a 1e-4 relative margin so that the FP32 test never
 //   drops a pair that the FP64 distance puts inside the list radius.
 // Per-step bins: every block also clears its cell in the other bin set (used by the next
@@ -544,7 +545,10 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const float Lf = (float)Ld, invLf = (float)(1.0 / Ld);
+    // the squared list radius in the units of the test: length, or L / 2^32 for the
+    // fixed-point positions
+    const float r2test = SHIFT ? rlist2f
+                               : (float)((double)rlist2f * (4294967296.0 / Ld) * (4294967296.0 / Ld));
     // `parts` blocks share a home cell, each taking a contiguous slice of its atoms (more
     // blocks in flight when the cells are few and large, as for water)
     const int cid = blockIdx.x / parts, part = blockIdx.x % parts;
@@ -602,7 +606,13 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             const int i = atoms[(size_t)cid * ccap + own0 + t];
             const double4 p = pos[i];
             okey[t] = i;
-            opos[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+            if (SHIFT) {
+                opos[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+            } else {   // the fixed-point box fractions (their bits), minimum image by integers
+                const uint4 qv = mx_posq(pos, n)[i];
+                opos[t] = make_float4(__uint_as_float(qv.x), __uint_as_float(qv.y),
+                                      __uint_as_float(qv.z), 0.0f);
+            }
             ocnt[t] = 0;
             if (rotate) {   // the bin of the atom's current position, for the next build
                 const int c2 = (mx_cell_coord(p.z, inv_cs, nc) * nc +
@@ -637,7 +647,13 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             // the barrier and the rank counts below
             constexpr int QN = BUILD_SLOTS / BUILD_THREADS;
             int ck[QN], cs[QN], cr[QN];
-            double4 pg[QN];
+            // per-step bins: FP64 positions shifted to the image next to the home cell;
+            // other bins: the 16-byte fixed-point positions
+            using gather_t = typename std::conditional<SHIFT, double4, uint4>::type;
+            const gather_t* gsrc;
+            if constexpr (SHIFT) gsrc = pos;
+            else gsrc = mx_posq(pos, n);
+            gather_t pg[QN];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 const int t = tid + q * BUILD_THREADS;
@@ -652,7 +668,7 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
             }
 #pragma unroll
             for (int q = 0; q < QN; ++q)
-                if (ck[q] >= 0) pg[q] = pos[ck[q]];
+                if (ck[q] >= 0) pg[q] = gsrc[ck[q]];
             __syncthreads();
             // rank by index within the candidate's cell (the cell's indices read four at
             // a time where aligned), so that candidates are stored in list order
@@ -672,15 +688,20 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
                 }
             }
             // positions at the candidate's image next to the home cell (FP64 sum, one FP32
-            // rounding)
+            // rounding), or the fixed-point fractions' bits
             float4 cp[QN];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 if (ck[q] >= 0) {
-                    const float4 sh = SHIFT ? segshift[cs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    cp[q] = make_float4((float)(pg[q].x + (double)sh.x * Ld),
-                                        (float)(pg[q].y + (double)sh.y * Ld),
-                                        (float)(pg[q].z + (double)sh.z * Ld), 0.0f);
+                    if constexpr (SHIFT) {
+                        const float4 sh = segshift[cs[q]];
+                        cp[q] = make_float4((float)(pg[q].x + (double)sh.x * Ld),
+                                            (float)(pg[q].y + (double)sh.y * Ld),
+                                            (float)(pg[q].z + (double)sh.z * Ld), 0.0f);
+                    } else {
+                        cp[q] = make_float4(__uint_as_float(pg[q].x), __uint_as_float(pg[q].y),
+                                            __uint_as_float(pg[q].z), 0.0f);
+                    }
                 }
             }
             __syncthreads();
@@ -720,14 +741,18 @@ k_build_cells(const double4* __restrict__ pos, const double* __restrict__ x, int
 #pragma unroll
                     for (int k = 0; k < BUILD_OWN_BATCH; ++k) {
                         if (k >= no) break;   // scalar
-                        float dx = pi[k].x - pj.x, dy = pi[k].y - pj.y, dz = pi[k].z - pj.z;
-                        if (!SHIFT) {
-                            dx -= Lf * rintf(dx * invLf);
-                            dy -= Lf * rintf(dy * invLf);
-                            dz -= Lf * rintf(dz * invLf);
+                        float dx, dy, dz;
+                        if constexpr (SHIFT) {
+                            dx = pi[k].x - pj.x;
+                            dy = pi[k].y - pj.y;
+                            dz = pi[k].z - pj.z;
+                        } else {   // minimum image in units of L / 2^32
+                            dx = (float)(int)(__float_as_uint(pi[k].x) - __float_as_uint(pj.x));
+                            dy = (float)(int)(__float_as_uint(pi[k].y) - __float_as_uint(pj.y));
+                            dz = (float)(int)(__float_as_uint(pi[k].z) - __float_as_uint(pj.z));
                         }
                         // every term evaluated (no short-circuit branches)
-                        const bool hit = v & (dx * dx + dy * dy + dz * dz < rlist2f) &
+                        const bool hit = v & (dx * dx + dy * dy + dz * dz < r2test) &
                                          (j != ii[k]) & !(WATER && jm == im[k]);
                         const unsigned long long mask = __ballot(hit);
                         if (mask == 0ull) continue;   // scalar
