@@ -28,7 +28,8 @@
 //
 // Pair law (per ordered pair, FP32): r2 = |r|^2; LJ (2 s^12 - s^6)/r2 * r with
 // s^2 = sigma^2/r2; Coulomb q_j / r^3 * r; RF q_j (1/r^3 - 2 k_rf) * r.  Per-atom
-// sums use Kahan compensation per lane; lanes are combined in FP64.
+// sums use Kahan compensation per lane (list kernel of charged plans: the plain FP32 sum
+// of each pass of NB_UNROLL pairs is the compensated term); lanes are combined in FP64.
 #include "nbx_mixed.h"
 
 #define NB_THREADS 256
@@ -61,6 +62,37 @@ __device__ __forceinline__ void nb_pair(float rx, float ry, float rz, float qj, 
             acc[3].add(f * rx);
             acc[4].add(f * ry);
             acc[5].add(f * rz);
+        }
+    }
+}
+
+// The same pair law added into plain FP32 sums g[0..5] (the list kernel of charged
+// plans sums a pass of NB_UNROLL pairs this way and adds the pass sum to its Kahan sums:
+// the compensation once per pass instead of once per pair)
+template <bool LJ, int EL>
+__device__ __forceinline__ void nb_pair_sum(float rx, float ry, float rz, float qj,
+                                            bool lj_pair, const mx_pair_params& pp,
+                                            float* g) {
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (LJ) {
+        if (lj_pair && r2 < pp.lj_R2) {
+            const float inv = __builtin_amdgcn_rcpf(r2);
+            const float q = pp.lj_s2 * inv;
+            const float s6 = q * q * q;
+            const float f = (2.0f * s6 * s6 - s6) * inv;
+            g[0] = fmaf(f, rx, g[0]);
+            g[1] = fmaf(f, ry, g[1]);
+            g[2] = fmaf(f, rz, g[2]);
+        }
+    }
+    if (EL != 0) {
+        if (r2 < pp.el_R2) {
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r3 = rinv * rinv * rinv;
+            const float f = (EL == 2) ? qj * (r3 - pp.krf2) : qj * r3;
+            g[3] = fmaf(f, rx, g[3]);
+            g[4] = fmaf(f, ry, g[4]);
+            g[5] = fmaf(f, rz, g[5]);
         }
     }
 }
@@ -147,7 +179,17 @@ k_nb_list(const double4* __restrict__ pos, const IDX* __restrict__ nbr,
         // gathers read the 16-byte fixed-point copy: integer minimum image, FP32 charge
         const uint4* qpos = mx_posq(pos, n);
         const uint4 qi = qpos[i];
-        const float sc = (float)(pp.L * 0x1p-32);
+        // displacements stay in units of L / 2^32 (one rounding, integer -> FP32); the pair
+        // law's constants carry the unit instead: sigma^2 and the cutoffs^2 divided by
+        // u^2, 2 k_rf times u^3 (u = L / 2^32).  The LJ sums are then 1/u, the
+        // electrostatic ones 1/u^2 times the forces, rescaled in FP64 below.  Every scaled
+        // quantity is a normal FP32 number: a displacement is below 2^31 units.
+        const double un = pp.L * 0x1p-32, iu = 1.0 / un;
+        mx_pair_params ps = pp;
+        ps.lj_s2 = (float)((double)pp.lj_s2 * iu * iu);
+        ps.lj_R2 = (float)((double)pp.lj_R2 * iu * iu);
+        ps.el_R2 = (float)((double)pp.el_R2 * iu * iu);
+        ps.krf2 = (float)((double)pp.krf2 * un * un * un);
         int jn[NB_UNROLL];
 #pragma unroll
         for (int u = 0; u < NB_UNROLL; ++u) jn[u] = lane + u * G < cnt ? (int)p[u * stride] : i;
@@ -162,19 +204,41 @@ k_nb_list(const double4* __restrict__ pos, const IDX* __restrict__ nbr,
             const int kn = k + NB_UNROLL * G;
 #pragma unroll
             for (int u = 0; u < NB_UNROLL; ++u) jn[u] = kn + u * G < cnt ? (int)p[u * stride] : i;
+            // charged plans (VALU bound: about 430 entries per water atom) add each pass of
+            // pairs in plain FP32 and compensate the pass sum; Lennard-Jones-only plans
+            // compensate every pair (their close-range terms are large: the pass sums
+            // measured 1.5e-6 against 8.5e-7 of the largest force at N = 32,000)
+            constexpr bool PASS_SUMS = EL != 0;
+            float g[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int u = 0; u < NB_UNROLL; ++u) {
                 if (k + u * G >= cnt) break;
-                const float rx = (float)(int)(qi.x - pq[u].x) * sc;
-                const float ry = (float)(int)(qi.y - pq[u].y) * sc;
-                const float rz = (float)(int)(qi.z - pq[u].z) * sc;
+                const float rx = (float)(int)(qi.x - pq[u].x);
+                const float ry = (float)(int)(qi.y - pq[u].y);
+                const float rz = (float)(int)(qi.z - pq[u].z);
                 const bool ljp = io_ && (!WATER || (jj[u] % 3) == 0);
-                nb_pair<LJ, EL>(rx, ry, rz, __uint_as_float(pq[u].w), ljp, pp, acc);
+                if constexpr (PASS_SUMS)
+                    nb_pair_sum<LJ, EL>(rx, ry, rz, __uint_as_float(pq[u].w), ljp, ps, g);
+                else
+                    nb_pair<LJ, EL>(rx, ry, rz, __uint_as_float(pq[u].w), ljp, ps, acc);
+            }
+            if constexpr (PASS_SUMS) {
+                if (LJ) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) acc[q].add(g[q]);
+                }
+#pragma unroll
+                for (int q = 3; q < 6; ++q) acc[q].add(g[q]);
             }
         }
         double F[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) F[q] = lane_sum<G>(acc[q].value());
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            F[q] *= iu;
+            F[3 + q] *= iu * iu;
+        }
         if (lane == 0) tval = nb_finish<LJ, EL, WATER>(i, n, F, A, pp, kp, io);
     }
     if (!WATER) mx_epilogue(tval, io, n);

@@ -4,8 +4,8 @@ With M > 1 only every M-th step launches the list build and the binning; a rebui
 requested earlier (at (skin/2)(1 - (M-1)/(2M))) and an atom that moves past skin/2 between
 two opportunities marks the chunk stale, which is then replayed with an opportunity at every
 step.  The bar: after a short run (100 steps) the positions deviate from the FP64
-reference-order path (whose arithmetic equals the oracle's) by no more than 1.1 times the
-deviation with M = 1.
+reference-order path (whose arithmetic equals the oracle's) by less than 1e-4 with every M
+(the reach of a cutoff crossing on a different step; a missed pair gives ~1e-3).
 """
 import numpy as np
 import pytest
@@ -48,10 +48,12 @@ def test_build_interval_follows_fp64_path(nbx, monkeypatch, cells):
         u, v = ctx.get_state()
         devs[every] = float(np.abs(u - ur).max())
     print(f"[case] max |du| vs FP64 after 100 steps: {devs}", flush=True)
-    # the deviation is that of the FP32 pair forces (grown by the chaotic dynamics of a
-    # hot jittered lattice): the same with every build interval
-    assert devs[1] < 1e-4
-    assert max(devs.values()) <= 1.1 * devs[1]
+    # the deviation is that of the FP32 pair forces, grown by the chaotic dynamics of a
+    # hot jittered lattice.  The build interval changes the order of the list entries and
+    # so the rounding of the FP32 sums; where that puts a pair's cutoff crossing on another
+    # step (the truncated force jumps by 0.039 at r = 2.5) an atom moves by up to ~1e-4
+    # after 100 steps, while a pair missing from a list moves atoms by ~1e-3
+    assert all(d < 1e-4 for d in devs.values()), devs
     # split runs reproduce an unsplit run bit for bit with M = 5 as well
     monkeypatch.setenv("NBX_BUILD_EVERY", "5")
     a = lj_ctx(s, _lib.PREC_MIXED, skin=0.5)
