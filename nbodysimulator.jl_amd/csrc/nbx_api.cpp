@@ -316,6 +316,7 @@ extern "C" int nbx_set_spcfw(nbx_ctx* c, double rOH, double aHOH, double kbond, 
 // the extra coordinate column of the reference's Nose-Hoover state starts at zero
 // (gather_bodies_initial_coordinates, src/nbody_to_ode.jl:1-20)
 static int reset_nosehoover(nbx_ctx* c) {
+    c->nh_advanced = false;
     NBX_HIP(c, hipMemsetAsync(c->d_scal + NBX_SCAL_ZETA, 0, 2 * sizeof(double), c->stream));
     NBX_HIP(c, hipStreamSynchronize(c->stream));
     return NBX_OK;

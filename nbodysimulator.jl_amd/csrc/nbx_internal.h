@@ -172,6 +172,10 @@ struct nbx_ctx {
     bool list_sorted = false;
     int64_t last_sort_step = -1;
     int64_t sorts_applied = 0;   // re-sorts that changed the order (diagnostics)
+    // The mixed path's last drift of a run (or chunk) is redone by the next one; its
+    // temperature finish advanced the Nose-Hoover zeta for the next step already, so the
+    // redone drift must not advance it again
+    bool nh_advanced = false;
     std::string err;
 };
 

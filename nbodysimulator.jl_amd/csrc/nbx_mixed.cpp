@@ -739,7 +739,9 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
                 kp.step_off = 0;
                 kp.ids = sorting ? c->d_perm : nullptr;
                 const int first_opp = (step0 % build_every(c)) == 0 ? 1 : 0;
-                const mx_io io0 = make_io(c, p, 0, 1, dt, first_opp);
+                mx_io io0 = make_io(c, p, 0, 1, dt, first_opp);
+                // a redone drift recomputes xi but leaves zeta as its first pass left it
+                if (c->nh_advanced) io0.nh.advance = 0;
                 mx_launch_drift(s, c, kp, io0);
                 if (io0.need_T) mx_launch_temperature_finish(s, io0);
                 if (first_opp) after_drift(c, p, 1);
@@ -771,6 +773,8 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
                     ++c->kt_launches;
                 }
                 c->kt_used = 0;
+                // the chunk's last drift advanced zeta for the step after the state
+                c->nh_advanced = true;
                 break;
             }
             c->kt_used = 0;   // replayed: its timings are dropped
@@ -1413,7 +1417,9 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     NBX_HIP(c, hipMemcpyAsync(bv.p, c->d_v, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipStreamSynchronize(s));
     const int64_t steps0 = c->steps_taken;
+    const bool nh0 = c->nh_advanced;
     auto restore = [&]() -> int {
+        c->nh_advanced = nh0;
         NBX_HIP(c, hipMemcpyAsync(c->d_x, bx.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipMemcpyAsync(c->d_v, bv.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipMemcpyAsync(c->d_scal, bs.p, sizeof(double) * 16, hipMemcpyDeviceToDevice, s));

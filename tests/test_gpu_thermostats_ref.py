@@ -92,6 +92,45 @@ def test_mixed_nosehoover_tracks_fp64(nbx):
         assert abs(T[-1] - target) < abs(T[0] - target)
 
 
+def test_mixed_nosehoover_split_runs_bitwise(nbx):
+    """The mixed path redoes the last drift of a run (or of a chunk) at the start of the
+    next one; the thermostat variable zeta must advance once per step all the same: runs of
+    40 + 60 (and 700 steps, two internal chunks) equal unsplit ones bit for bit, LJ and
+    SPC/Fw water."""
+    print("[case] mixed Nose-Hoover split runs N=864", flush=True)
+    a, _, s = lj_nh(6, _lib.PREC_MIXED)
+    b, _, _ = lj_nh(6, _lib.PREC_MIXED)
+    a.set_state(s.u, s.v)
+    b.set_state(s.u, s.v)
+    a.vv_run(0.005, 40)
+    a.vv_run(0.005, 60)
+    b.vv_run(0.005, 100)
+    ua, va = a.get_state()
+    ub, vb = b.get_state()
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+    print("[case] mixed Nose-Hoover 700 steps against 300 + 400, water", flush=True)
+    w = gen.water_box(per_side=8, seed=3)
+    P = gen.SPCFW
+    ctxs = []
+    for _ in range(2):
+        c = _lib.Context(len(w.masses), _lib.SYS_WATER, w.masses, w.charges)
+        c.set_boundary(_lib.BC_CUBIC, [w.L, 0, 0, 0, 0, 0])
+        c.set_kb(P["kb"])
+        c.add_lennard_jones(P["eps"], P["sigma"], 0.9)
+        c.add_reaction_field(P["k_el"], 0.9, math.inf)
+        c.set_spcfw(P["rOH"], P["aHOH"], P["kbond"], P["kangle"])
+        c.set_precision(_lib.PREC_MIXED)
+        c.set_thermostat(_lib.THERM_NOSEHOOVER, 300.0, 0.05)
+        c.set_state(w.u, w.v)
+        ctxs.append(c)
+    ctxs[0].vv_run(0.0005, 700)
+    ctxs[1].vv_run(0.0005, 300)
+    ctxs[1].vv_run(0.0005, 400)
+    u0, v0 = ctxs[0].get_state()
+    u1, v1 = ctxs[1].get_state()
+    assert np.array_equal(u0, u1) and np.array_equal(v0, v1)
+
+
 def argon(thermostat_factory, precision, alg_kind="vv"):
     T = 120.0
     T0 = 90
