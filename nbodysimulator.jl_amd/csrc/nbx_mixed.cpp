@@ -259,6 +259,20 @@ mx_nh_params nh_params(const nbx_ctx* c, double dt, int advance) {
     return nh;
 }
 
+// the temperature partials of the drift that writes X[par]: one buffer per parity (an int
+// count, then one partial per block; n/2 + 4,096 doubles each, more than any step kernel's
+// blocks), so a step can add the partials of the drift before it while its own drift
+// writes the other buffer
+double* red_buf(const nbx_ctx* c, int par) {
+    return c->d_red + (size_t)par * (size_t)(c->n / 2 + 4096);
+}
+
+// water with Berendsen: the molecule kernel adds the partials itself (kp.T_parts), and no
+// temperature finish runs between the steps
+bool folds_temperature(const nbx_ctx* c, const mx_plan& p) {
+    return p.kind == PLAN_WATER && c->therm == NBX_THERM_BERENDSEN && !c->em;
+}
+
 // buffers of a launch that reads x in X[par] and (when drifting) writes X[par ^ 1];
 // next_opp: the next step has a build opportunity
 mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0.0,
@@ -297,8 +311,9 @@ mx_io make_io(nbx_ctx* c, const mx_plan& p, int par, int do_drift, double dt = 0
     }
     io.need_T = ((c->therm == NBX_THERM_BERENDSEN || c->therm == NBX_THERM_NOSEHOOVER) && do_drift)
                     ? 1 : 0;
-    io.red_part = c->d_red;
-    io.red_count = c->nl.d_flags + 4;
+    double* rb = red_buf(c, par ^ 1);
+    io.red_count = reinterpret_cast<int*>(rb);
+    io.red_part = rb + 1;
     io.T_out = c->d_scal;
     io.T_denom = c->kb * (double)(3 * c->n - constraints(c));
     io.nh = nh_params(c, dt, 1);   // with need_T: the Nose-Hoover update of this drift
@@ -331,6 +346,8 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
                  bool timed = false, int next_opp = 1) {
     hipStream_t s = c->stream;
     const mx_io io = make_io(c, p, par, do_drift, kp.dt, next_opp);
+    // the step's kick takes the temperature of the drift before it from its partials
+    const bool fold = kp.do_kick && folds_temperature(c, p);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) timing_events(c, &e0, &e1);
     if (p.kind == PLAN_GRAVITY) {
@@ -343,9 +360,16 @@ void launch_step(nbx_ctx* c, const mx_plan& p, const mx_kick_params& kp, int par
         if (e0) (void)hipEventRecord(e0, s);
         mx_launch_nonbonded(s, nb, c, pbuf(c, par), p.pp, kp, io);
         if (e1) (void)hipEventRecord(e1, s);
-        if (p.kind == PLAN_WATER) mx_launch_water_molecules(s, c, kp, io);
+        if (p.kind == PLAN_WATER) {
+            mx_kick_params kw = kp;
+            if (fold) {
+                kw.T_parts = red_buf(c, par);
+                kw.T_denom = io.T_denom;
+            }
+            mx_launch_water_molecules(s, c, kw, io);
+        }
     }
-    if (io.need_T) mx_launch_temperature_finish(s, io);
+    if (io.need_T && !fold) mx_launch_temperature_finish(s, io);
     if (do_drift && next_opp) after_drift(c, p, par ^ 1);
 }
 
@@ -743,7 +767,7 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
                 // a redone drift recomputes xi but leaves zeta as its first pass left it
                 if (c->nh_advanced) io0.nh.advance = 0;
                 mx_launch_drift(s, c, kp, io0);
-                if (io0.need_T) mx_launch_temperature_finish(s, io0);
+                if (io0.need_T && !folds_temperature(c, p)) mx_launch_temperature_finish(s, io0);
                 if (first_opp) after_drift(c, p, 1);
             }
             const int gs = c->graph_steps;

@@ -42,6 +42,11 @@ struct mx_kick_params {
     const int64_t* step_base;  // if set, step = *step_base + step_off (graph replays)
     int64_t step_off;
     const double* d_T;    // Berendsen: temperature of the half-kicked velocities
+    // if set (water, Berendsen): the molecule kernel adds the block partials of m vh^2 of
+    // the drift before it (an int count at T_parts[0], partials from T_parts[1]) and
+    // divides by T_denom itself, instead of reading d_T
+    const double* T_parts;
+    double T_denom;
     // Euler-Maruyama steps of the Langevin SDE (src/nbody_to_ode.jl:567-598) instead of
     // velocity Verlet: x' = x + dt v, v' = v + dt (a - gamma v) + sigma sqrt(dt) xi
     int em;
@@ -299,18 +304,22 @@ __device__ __forceinline__ bool mx_therm_force(const mx_kick_params& kp) {
     return kp.therm == NBX_THERM_BERENDSEN || kp.therm == NBX_THERM_NOSEHOOVER;
 }
 
-__device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
-    if (kp.therm == NBX_THERM_NOSEHOOVER) return -kp.d_T[NBX_SCAL_ZETA];
-    const double T = kp.d_T[0];
+__device__ __forceinline__ double mx_berendsen_from_T(const mx_kick_params& kp, double T) {
     if (1.0 / T == __builtin_inf()) return kp.ber_gamma;
     return kp.ber_gamma * (kp.T / T - 1.0);
+}
+
+__device__ __forceinline__ double mx_berendsen_coeff(const mx_kick_params& kp) {
+    if (kp.therm == NBX_THERM_NOSEHOOVER) return -kp.d_T[NBX_SCAL_ZETA];
+    return mx_berendsen_from_T(kp, kp.d_T[0]);
 }
 
 // Epilogue of a step kernel, called by every thread of every block (blockDim.x <= 256):
 // with io.need_T, the block sum of `val` (m vh^2) goes to red_part[blockIdx.x] and block 0
 // records the number of blocks in *red_count.  mx_launch_temperature_finish, launched next
 // on the stream, adds the partials in block order, writes the temperature of vh_{k+1} and
-// applies the Nose-Hoover update.  A separate launch instead of a last-block reduction:
+// applies the Nose-Hoover update; with Berendsen, the water molecule kernel of the next
+// step adds them itself instead (kp.T_parts).  Not a last-block reduction:
 // the kernel boundary makes every partial visible, where a device-scope fence per block
 // (writing back its XCD's L2) cost more than the launch.  Deterministic: the sum does not
 // depend on the order in which blocks finish.
@@ -330,6 +339,19 @@ __device__ __forceinline__ void mx_epilogue(double val, const mx_io& io, int n) 
         io.red_part[blockIdx.x] = red[0];
         if (blockIdx.x == 0) *io.red_count = (int)gridDim.x;
     }
+}
+
+// The same with one partial per wavefront of 64 consecutive atoms (grid thread g is atom g,
+// blockDim.x a multiple of 64; count ceil(n / 64)), summed by a butterfly: the partials do
+// not depend on the kernel's block size, so a drift redone at the start of a run (the drift
+// kernel) gives the temperature that the step kernel's drift gave, bit for bit
+__device__ __forceinline__ void mx_epilogue_waves(double val, const mx_io& io, int n) {
+    if (!io.need_T) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off, 64);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((g & 63) == 0 && g < n) io.red_part[g >> 6] = val;
+    if (g == 0) *io.red_count = (n + 63) >> 6;
 }
 
 // launchers (nbx_nonbonded.hip / nbx_gravity.hip)

@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(256) k_mx_drift(int n, mx_kick_params kp, mx_i
         const double v3[3] = {io.v[i], io.v[n + i], io.v[2 * n + i]};
         tval = mx_drift_atom(i, n, v3, io.a[i], io.a[n + i], io.a[2 * n + i], A, kp, io);
     }
-    mx_epilogue(tval, io, n);
+    mx_epilogue_waves(tval, io, n);
 }
 
 void mx_launch_drift(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,
@@ -995,11 +995,26 @@ __device__ __forceinline__ void mol_bond(const double* ri, const double* rj, dou
     f[2] += fac * rz;
 }
 
+// sum over the block's 3 * WATER_MOLS threads, the same value in every thread and the same
+// order in every block: a butterfly within each wavefront (a + b == b + a, so every lane
+// ends with the same sum), then the three wavefront sums in order
+#define WATER_MOLS 64
+__device__ __forceinline__ double water_block_sum(double s) {
+    __shared__ double ws[3];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return (ws[0] + ws[1]) + ws[2];
+}
+
 // One thread per atom, 64 molecules (192 atoms) per block: the block's positions are
 // shared through LDS, every thread evaluates its molecule's bonds and angle (the same
 // expressions in the same order for all three) and keeps its own atom's terms, then ends
 // its atom's step.  Andersen draws one uniform per molecule (counter of its O atom).
-#define WATER_MOLS 64
+// With kp.T_parts (Berendsen) every block adds the partials of m vh^2 that the drift
+// before this step left (in block order of that drift), so no kernel between the steps
+// finishes the temperature.
 __global__ void __launch_bounds__(3 * WATER_MOLS)
 k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_params kp,
               mx_io io) {
@@ -1018,9 +1033,24 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
 #pragma unroll
         for (int k = 0; k < 3; ++k) anb[k] = io.a[(size_t)k * n + i];
     }
+    // the partial count and one partial per thread are requested with the atom's inputs
+    // (the buffer holds at least 4,095 partials, so partial t is in bounds)
+    const bool fold = kp.T_parts != nullptr;
+    int tnb = 0;
+    double tpart = 0.0;
+    if (fold) {
+        tnb = *reinterpret_cast<const int*>(kp.T_parts);
+        tpart = kp.T_parts[1 + t];
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) sx[k][t] = act ? io.x_in[(size_t)k * n + i] : 0.0;
     __syncthreads();
+    double Tb = 0.0;
+    if (fold) {
+        double ps = t < tnb ? tpart : 0.0;
+        for (int b = t + 3 * WATER_MOLS; b < tnb; b += 3 * WATER_MOLS) ps += kp.T_parts[1 + b];
+        Tb = water_block_sum(ps) / kp.T_denom;
+    }
     double tval = 0.0;
     if (act) {
         const int w = t % 3;               // 0: O, 1: H1, 2: H2
@@ -1074,7 +1104,7 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
             else acc[k] += fc * im;
         }
         if (mx_therm_force(kp)) {
-            const double cb = mx_berendsen_coeff(kp);
+            const double cb = fold ? mx_berendsen_from_T(kp, Tb) : mx_berendsen_coeff(kp);
             for (int k = 0; k < 3; ++k) acc[k] += cb * A.vh[k];
         }
         for (int k = 0; k < 3; ++k) io.a[(size_t)k * n + i] = acc[k];
@@ -1090,7 +1120,7 @@ k_water_atoms(int n, double rOH, double kbond, double aHOH, double ka, mx_kick_p
             if (io.do_drift) tval = mx_drift_atom(i, n, v3, acc[0], acc[1], acc[2], A, kp, io);
         }
     }
-    mx_epilogue(tval, io, n);
+    mx_epilogue_waves(tval, io, n);
 }
 
 void mx_launch_water_molecules(hipStream_t s, const nbx_ctx* c, const mx_kick_params& kp,

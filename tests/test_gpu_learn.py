@@ -199,6 +199,54 @@ def test_drift_monitor_falls_back_to_plain_vv(nbx):
     assert ctx.steps_taken() == 240
 
 
+def test_learned_stride_at_the_benchmark_size(nbx):
+    """The learned part at LJ N=32,000 (FCC 20^3, rho* 0.8442): collection (targets of the
+    stride-1 step vanish), the zero network as plain velocity Verlet at the stride, and the
+    fallback to plain steps, each against plain runs of the same context type."""
+    print("[case] learned part at N=32000: collect", flush=True)
+    ctx, s = lj_ctx(20, 0.9, seed=21, jitter=0.02)
+    m = _lib.Model(0)
+    ctx.learn_collect(m, 0.005, 1, 3, 256, seed=4)
+    f, t = m.get_samples()
+    assert f.shape == (768, 35) and np.all(np.isfinite(f))
+    assert np.abs(t).max() <= 1e-12
+    m.clear_samples()
+    ctx.learn_collect(m, 0.005, 8, 4, 256, seed=5)
+    f, t = m.get_samples()
+    assert f.shape == (1024, 35) and np.all(np.isfinite(t))
+    assert ctx.steps_taken() == 3 + 32
+    print("[case] learned part at N=32000: zero network", flush=True)
+    ctx, s = lj_ctx(20, 0.9, seed=22, jitter=0.02)
+    ref, _ = lj_ctx(20, 0.9, seed=22, jitter=0.02)
+    nm = np.concatenate([np.zeros(35), np.ones(35), np.ones(6)])
+    m.set(np.zeros(M.NPARAM), nm)
+    rep = ctx.learned_run(m, 0.005, 4, 20, drift_tol=1.0, check_every=5)
+    assert rep["strided_steps"] == 20 and not rep["tripped"] and rep["plain_steps"] == 0
+    ref.vv_run(0.02, 20)
+    u1, v1 = ctx.get_state()
+    u2, v2 = ref.get_state()
+    du, dv = np.abs(u1 - u2).max(), np.abs(v1 - v2).max()
+    print(f"[case] zero network vs plain VV at 4 dt: |du| {du:.2e} |dv| {dv:.2e}", flush=True)
+    # the lists are rebuilt at other moments than in the plain run, which reorders the FP32
+    # pair sums (~1e-7 of a); 20 steps of the fluid grow that to well under these bars
+    assert du <= 1e-5 and dv <= 1e-4
+    ke, pe = ctx.learn_energy()
+    assert abs((ke + pe) - rep["E"]) <= 1e-9 * abs(rep["E"])
+    print("[case] learned part at N=32000: fallback", flush=True)
+    ctx, s = lj_ctx(20, 0.9, seed=23, jitter=0.02)
+    ref, _ = lj_ctx(20, 0.9, seed=23, jitter=0.02)
+    p, _ = random_params(9)
+    m.set(p, np.concatenate([np.zeros(35), np.ones(35), np.full(6, 1e-3)]))
+    rep = ctx.learned_run(m, 0.005, 8, 10, drift_tol=1e-4, check_every=10)
+    assert rep["tripped"] and rep["strided_before_trip"] == 0
+    assert rep["strided_steps"] == 0 and rep["plain_steps"] == 80
+    ref.vv_run(0.005, 80)
+    u1, _ = ctx.get_state()
+    u2, _ = ref.get_state()
+    assert np.abs(u1 - u2).max() <= 1e-5
+    assert ctx.steps_taken() == 80
+
+
 def test_trained_network_beats_the_uncorrected_stride(nbx):
     """The network's gain: on held-out trajectories its corrected 8-dt step is closer to
     eight plain steps (the yardstick) than the uncorrected 8-dt step, along the force."""

@@ -131,6 +131,53 @@ def test_mixed_nosehoover_split_runs_bitwise(nbx):
     assert np.array_equal(u0, u1) and np.array_equal(v0, v1)
 
 
+def water_berendsen(w, prec, T0, tau):
+    P = gen.SPCFW
+    c = _lib.Context(len(w.masses), _lib.SYS_WATER, w.masses, w.charges)
+    c.set_boundary(_lib.BC_CUBIC, [w.L, 0, 0, 0, 0, 0])
+    c.set_kb(P["kb"])
+    c.add_lennard_jones(P["eps"], P["sigma"], 0.9)
+    c.add_reaction_field(P["k_el"], 0.9, math.inf)
+    c.set_spcfw(P["rOH"], P["aHOH"], P["kbond"], P["kangle"])
+    c.set_precision(prec)
+    c.set_thermostat(_lib.THERM_BERENDSEN, T0, tau)
+    c.set_state(w.u, w.v)
+    return c
+
+
+def test_mixed_water_berendsen_follows_fp64_and_splits_bitwise(nbx):
+    """The mixed water step takes the Berendsen temperature of the drift before it from
+    that drift's block partials (no kernel runs between the steps).  Strong coupling
+    (tau 0.01 ps, T0 150 K from 300 K: the coefficient gamma (T0/T - 1) starts at -25/ps and
+    T drops by ~7 K per step at first) makes the trajectory depend on that temperature
+    step by step; the mixed trajectory stays within 1e-5 nm of the FP64 path's (the bar of the full-size water test), its
+    temperature within 1e-5, and runs of 300 + 400 steps (chunks of at most 500, graphs
+    of 50) equal a 700-step run bit for bit."""
+    print("[case] mixed water Berendsen vs FP64, 512 molecules", flush=True)
+    w = gen.water_box(per_side=8, seed=3)
+    mx = water_berendsen(w, _lib.PREC_MIXED, 150.0, 0.01)
+    rf = water_berendsen(w, _lib.PREC_FP64_REF, 150.0, 0.01)
+    mx.vv_run(0.0005, 200)
+    rf.vv_run(0.0005, 200)
+    um, _ = mx.get_state()
+    ur, _ = rf.get_state()
+    dev = float(np.abs(um - ur).max())
+    Tm, Tr = mx.temperature(), rf.temperature()
+    print(f"[case] max |du| {dev:.2e} nm, T {Tm:.4f} / {Tr:.4f} K", flush=True)
+    assert dev < 1e-5
+    assert abs(Tm - Tr) <= 1e-5 * Tr
+    assert Tr < 280.0   # the coupling is strong enough to matter
+    print("[case] mixed water Berendsen 700 steps against 300 + 400", flush=True)
+    a = water_berendsen(w, _lib.PREC_MIXED, 150.0, 0.01)
+    b = water_berendsen(w, _lib.PREC_MIXED, 150.0, 0.01)
+    a.vv_run(0.0005, 700)
+    b.vv_run(0.0005, 300)
+    b.vv_run(0.0005, 400)
+    ua, va = a.get_state()
+    ub, vb = b.get_state()
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+
+
 def argon(thermostat_factory, precision, alg_kind="vv"):
     T = 120.0
     T0 = 90
