@@ -1507,12 +1507,12 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         best1 = c->nl.skin;
         best2 = build_every(c);
         const int lanes_c[5] = {2, 4, 8, 16, 32};
-        const double skin_c[7] = {0.06, 0.09, 0.12, 0.18, 0.24, 0.32, 0.40};
+        const double skin_c[8] = {0.06, 0.09, 0.12, 0.15, 0.18, 0.24, 0.32, 0.40};
         const int every_c[3] = {1, 2, 5};
         bool set_best = false;
         for (int ei = 0; ei < 3; ++ei)
             for (int li = 0; li < 5; ++li)
-                for (int si = 0; si < 7; ++si) {
+                for (int si = 0; si < 8; ++si) {
                     c->nl.lanes = lanes_c[li];
                     c->skin = skin_c[si] * p.cutoff;
                     c->skin_set = true;
