@@ -123,7 +123,7 @@ extern "C" int nbx_create(nbx_ctx** out, int64_t n, int32_t system_kind, const d
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_ms, b1);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_qs, b1);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_red, b1 + 8192 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&c->d_scal, 16 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_scal, NBX_SCAL_COUNT * sizeof(double));
     const size_t bi = sizeof(int) * (size_t)(n > 0 ? n : 1);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_perm, bi);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_iperm, bi);
@@ -400,12 +400,16 @@ extern "C" int nbx_get_state(const nbx_ctx* c, double* u, double* v) {
 
 extern "C" int64_t nbx_steps_taken(const nbx_ctx* c) { return c ? c->steps_taken : -1; }
 
-// Nose-Hoover extended coordinate and its rate (the reference's u[3n+1] and v[3n+1])
+// Nose-Hoover extended coordinate and its rate (the reference's u[3n+1] and v[3n+1]) at
+// the state; after a mixed run the last drift has already advanced them for the next step,
+// and the values before that advance are the state's
 extern "C" int nbx_get_thermostat_state(const nbx_ctx* c, double* zeta, double* xi) {
     if (!c) return NBX_ERR_INVALID;
     nbx_ctx* m = const_cast<nbx_ctx*>(c);
     double h[2] = {0.0, 0.0};
-    NBX_HIP(m, hipMemcpyAsync(h, c->d_scal + NBX_SCAL_ZETA, sizeof(h), hipMemcpyDeviceToHost,
+    const bool ahead = c->nh_advanced && c->therm == NBX_THERM_NOSEHOOVER;
+    const int slot = ahead ? NBX_SCAL_ZETA_PREV : NBX_SCAL_ZETA;
+    NBX_HIP(m, hipMemcpyAsync(h, c->d_scal + slot, sizeof(h), hipMemcpyDeviceToHost,
                               c->stream));
     NBX_HIP(m, hipStreamSynchronize(c->stream));
     if (zeta) *zeta = h[0];
@@ -580,6 +584,13 @@ static int ref_vv(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64_t 
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     int rc;
+    if (c->nh_advanced && c->therm == NBX_THERM_NOSEHOOVER) {
+        // a mixed run before this one left zeta and xi advanced for the next step: back to
+        // the state's values, which this path advances at its first drift
+        NBX_HIP(c, hipMemcpyAsync(c->d_scal + NBX_SCAL_ZETA, c->d_scal + NBX_SCAL_ZETA_PREV,
+                                  2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    c->nh_advanced = false;
     if (!c->accel_valid) {
         if ((rc = ref_rhs(c, c->d_x, c->d_v, c->d_a, c->d_scal + NBX_SCAL_XI))) return rc;
         c->accel_valid = true;

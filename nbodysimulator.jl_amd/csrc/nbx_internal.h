@@ -13,7 +13,10 @@
 // device scalars (nbx_ctx::d_scal): [0] thermostat temperature, [1..7] energies, [8] finite
 // flag, [9] temperature, [10] Nose-Hoover zeta, [11] its rate xi, [12] pair count,
 // [13..14] list energies, [15] scratch rate for state-free RHS calls
-enum { NBX_SCAL_T = 0, NBX_SCAL_ZETA = 10, NBX_SCAL_XI = 11, NBX_SCAL_XI_SCRATCH = 15 };
+// ZETA_PREV / XI_PREV: zeta and xi before the mixed path's last advance (that advance
+// belongs to the drift of the step after the state when a run has ended)
+enum { NBX_SCAL_T = 0, NBX_SCAL_ZETA = 10, NBX_SCAL_XI = 11, NBX_SCAL_XI_SCRATCH = 15,
+       NBX_SCAL_ZETA_PREV = 16, NBX_SCAL_XI_PREV = 17, NBX_SCAL_COUNT = 32 };
 enum { NBX_BINS_STEP = 0, NBX_BINS_FIXED = 1, NBX_BINS_BUILD = 2, NBX_FLAG_BINSET = 5 };
 
 enum { NBX_TERM_LJ = 1, NBX_TERM_ELEC = 2, NBX_TERM_GRAV = 3, NBX_TERM_RF = 4, NBX_TERM_MAG = 5 };

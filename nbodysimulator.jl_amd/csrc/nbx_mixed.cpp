@@ -650,7 +650,7 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     backup4 bk;
-    NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * (4 * n3 + 16)));
+    NBX_HIP(c, hipMalloc((void**)&bk.p, sizeof(double) * (4 * n3 + NBX_SCAL_COUNT)));
     const int64_t chunk_max = 500;
     while (c->kt_on && c->kt_ev.size() < 2 * (size_t)chunk_max) {
         hipEvent_t e = nullptr;
@@ -743,7 +743,7 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
         for (int k = 0; k < 4; ++k)
             NBX_HIP(c, hipMemcpyAsync(bk.p + k * n3, src[k], sizeof(double) * n3,
                                       hipMemcpyDeviceToDevice, s));
-        NBX_HIP(c, hipMemcpyAsync(bk.p + 4 * n3, c->d_scal, sizeof(double) * 16,
+        NBX_HIP(c, hipMemcpyAsync(bk.p + 4 * n3, c->d_scal, sizeof(double) * NBX_SCAL_COUNT,
                                   hipMemcpyDeviceToDevice, s));
         const int64_t step0 = c->steps_taken;
         for (int attempt = 0;; ++attempt) {
@@ -808,7 +808,7 @@ static int mx_vv_run(nbx_ctx* c, double dt, int64_t nsteps, uint64_t seed, int64
             for (int k = 0; k < 4; ++k)
                 NBX_HIP(c, hipMemcpyAsync(dst[k], bk.p + k * n3, sizeof(double) * n3,
                                           hipMemcpyDeviceToDevice, s));
-            NBX_HIP(c, hipMemcpyAsync(c->d_scal, bk.p + 4 * n3, sizeof(double) * 16,
+            NBX_HIP(c, hipMemcpyAsync(c->d_scal, bk.p + 4 * n3, sizeof(double) * NBX_SCAL_COUNT,
                                       hipMemcpyDeviceToDevice, s));
             if (h[1]) {
                 if ((rc = grow_lists(c, p, grow))) return rc;
@@ -1435,8 +1435,8 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
     int rc;
     if ((rc = dalloc(c, bx, sizeof(double) * n3))) return rc;
     if ((rc = dalloc(c, bv, sizeof(double) * n3))) return rc;
-    if ((rc = dalloc(c, bs, sizeof(double) * 16))) return rc;
-    NBX_HIP(c, hipMemcpyAsync(bs.p, c->d_scal, sizeof(double) * 16, hipMemcpyDeviceToDevice, s));
+    if ((rc = dalloc(c, bs, sizeof(double) * NBX_SCAL_COUNT))) return rc;
+    NBX_HIP(c, hipMemcpyAsync(bs.p, c->d_scal, sizeof(double) * NBX_SCAL_COUNT, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipMemcpyAsync(bx.p, c->d_x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipMemcpyAsync(bv.p, c->d_v, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
     NBX_HIP(c, hipStreamSynchronize(s));
@@ -1446,7 +1446,7 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         c->nh_advanced = nh0;
         NBX_HIP(c, hipMemcpyAsync(c->d_x, bx.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipMemcpyAsync(c->d_v, bv.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
-        NBX_HIP(c, hipMemcpyAsync(c->d_scal, bs.p, sizeof(double) * 16, hipMemcpyDeviceToDevice, s));
+        NBX_HIP(c, hipMemcpyAsync(c->d_scal, bs.p, sizeof(double) * NBX_SCAL_COUNT, hipMemcpyDeviceToDevice, s));
         NBX_HIP(c, hipStreamSynchronize(s));
         c->steps_taken = steps0;
         c->accel_valid = false;

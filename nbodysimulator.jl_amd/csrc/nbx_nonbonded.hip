@@ -517,8 +517,12 @@ __global__ void __launch_bounds__(256) k_temperature_finish(mx_io io) {
         const double T = red[0] / io.T_denom;
         io.T_out[0] = T;
         if (io.nh.nh) {
-            if (io.nh.advance)
-                io.T_out[NBX_SCAL_ZETA] = io.T_out[NBX_SCAL_ZETA] + io.nh.dt * io.T_out[NBX_SCAL_XI];
+            if (io.nh.advance) {
+                const double z0 = io.T_out[NBX_SCAL_ZETA], x0 = io.T_out[NBX_SCAL_XI];
+                io.T_out[NBX_SCAL_ZETA_PREV] = z0;
+                io.T_out[NBX_SCAL_XI_PREV] = x0;
+                io.T_out[NBX_SCAL_ZETA] = z0 + io.nh.dt * x0;
+            }
             const double it = 1.0 / io.nh.tau;
             io.T_out[NBX_SCAL_XI] =
                 (it * it) * (T / io.nh.T0 - (double)(io.nh.ndf + 1) / (double)io.nh.ndf);

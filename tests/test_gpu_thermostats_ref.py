@@ -131,6 +131,35 @@ def test_mixed_nosehoover_split_runs_bitwise(nbx):
     assert np.array_equal(u0, u1) and np.array_equal(v0, v1)
 
 
+def test_mixed_nosehoover_reports_the_state_pair(nbx):
+    """The mixed path's last drift of a run belongs to the next step and advances zeta
+    already; nbx_get_thermostat_state reports the pair of the state all the same.  One step
+    from a fresh state: zeta_1 = dt xi_0, xi_0 = (1/tau)^2 (T(v_0)/T0 - (ndf+1)/ndf) (the
+    first right-hand side sees v_0; src/thermostats.jl:116-122).  A following FP64 run (a
+    new solve: its first right-hand side sees v) advances the reported zeta once per step:
+    zeta' = zeta + dt xi(T(v)) after one step."""
+    print("[case] mixed Nose-Hoover reported zeta, LJ N=864", flush=True)
+    T0, tau, dt, n = 0.5, 0.1, 0.005, 864
+    ctx, _, s = lj_nh(6, _lib.PREC_MIXED, T0=T0, tau=tau)
+    ctx.set_state(s.u, s.v)
+
+    def rate(T):
+        return (1.0 / tau) ** 2 * (T / T0 - (3 * n + 1) / (3 * n))
+
+    xi0 = rate(ctx.temperature())
+    ctx.vv_run(dt, 1)
+    z1, _ = ctx.thermostat_state()
+    assert abs(z1 - dt * xi0) <= 1e-12 * abs(dt * xi0), (z1, dt * xi0)
+    ctx.vv_run(dt, 39)
+    z40, _ = ctx.thermostat_state()
+    T40 = ctx.temperature()
+    ctx.set_precision(_lib.PREC_FP64_REF)
+    ctx.vv_run(dt, 1)
+    z41, _ = ctx.thermostat_state()
+    step = dt * rate(T40)
+    assert abs(z41 - (z40 + step)) <= 1e-9 * abs(step), (z41, z40, step)
+
+
 def water_berendsen(w, prec, T0, tau):
     P = gen.SPCFW
     c = _lib.Context(len(w.masses), _lib.SYS_WATER, w.masses, w.charges)
