@@ -1482,7 +1482,10 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
         best_ms = base_ms;
         best0 = tpt0;
         best1 = ch0;
-        for (int tpt = 1; tpt <= 4; tpt *= 2)
+        // first pass: the best chunk count of each tile width (targets per thread)
+        double wms[3] = {1e300, 1e300, 1e300};
+        int wch[3] = {0, 0, 0};
+        for (int k = 0, tpt = 1; tpt <= 4; tpt *= 2, ++k)
             for (int ch = 1; ch <= 64; ch *= 2) {
                 if (n / ch < 256) break;
                 c->grav_tpt_set = tpt;
@@ -1490,8 +1493,21 @@ int mx_autotune(nbx_ctx* c, double dt, int trial, double* out, int nout) {
                 double ms;
                 if ((rc = time_config(&ms))) return rc;
                 ++count;
-                if (ms < best_ms) { best_ms = ms; best0 = tpt; best1 = ch; }
+                if (ms < wms[k]) { wms[k] = ms; wch[k] = ch; }
             }
+        // second pass: those shapes twice more each, the fastest repeat kept (one trial
+        // varies by a few percent between runs, about the gap between the tile widths)
+        for (int k = 0, tpt = 1; tpt <= 4; tpt *= 2, ++k) {
+            if (!wch[k]) continue;
+            c->grav_tpt_set = tpt;
+            c->grav_chunks_set = wch[k];
+            for (int rep = 0; rep < 2; ++rep) {
+                double ms;
+                if ((rc = time_config(&ms))) return rc;
+                ++count;
+                if (ms < best_ms) { best_ms = ms; best0 = tpt; best1 = wch[k]; }
+            }
+        }
         c->grav_tpt_set = (int)best0;
         c->grav_chunks_set = (int)best1;
         if (!c->grav_tpt_set) best0 = n >= 16384 ? 2 : 1;
